@@ -1,0 +1,255 @@
+/*
+ * simlab_cfd.h — C ABI of the MI355X (gfx950) D3Q19 wind-tunnel lattice.
+ *
+ * Drop-in boundary for the uniform-grid lattice of SimLab's `crates/cfd`
+ * (crates/cfd/src/lattice.rs:220-841). The Rust `Lattice` type forwards each
+ * of its methods to one function here; the Rust side of that binding is shown
+ * in INTEGRATION.md. Everything is in lattice units (cell size 1, time step
+ * 1, reference density 1), as in the reference.
+ *
+ * Conventions
+ *  - Every function returns SIMLAB_OK (0) or an error code; the message of
+ *    the last error on the calling thread is `simlab_cfd_last_error()`. The
+ *    messages of setup errors are the reference's `bail!` strings.
+ *  - No C++ exception crosses this boundary. Inputs are copied at `_new`;
+ *    outputs go to caller-allocated arrays whose sizes `_info` reports.
+ *  - One HIP stream per lattice handle. Distinct handles may be used from
+ *    distinct threads; calls on one handle must be serialised by the caller
+ *    (as `&mut self` does in Rust).
+ */
+#ifndef SIMLAB_CFD_H
+#define SIMLAB_CFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMLAB_CFD_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    SIMLAB_OK = 0,
+    SIMLAB_ERR_INVALID = 1,   /* bad setup / arguments (reference `bail!`)  */
+    SIMLAB_ERR_GPU = 2,       /* HIP runtime failure, no device             */
+    SIMLAB_ERR_OOM = 3,       /* "not enough GPU memory for this grid"      */
+    SIMLAB_ERR_STATE = 4      /* call not valid in this state (e.g. macros
+                                 without debug_dump)                         */
+};
+
+/* Population storage. lattice.rs:11-32 has Fp32 and Fp16 (scaled IEEE half,
+ * stochastic rounding); Bf16 and Fp64 are additions of this library: Bf16
+ * stores the same deviations from the weights as bfloat16 with stochastic
+ * rounding, Fp64 is the double-precision oracle storage (FP64 arithmetic). */
+typedef enum {
+    SIMLAB_STORAGE_FP32 = 0,
+    SIMLAB_STORAGE_FP16 = 1,
+    SIMLAB_STORAGE_BF16 = 2,
+    SIMLAB_STORAGE_FP64 = 3
+} simlab_storage;
+
+/* Face boundary condition, lattice.rs:34-43 (`FaceBc`). */
+typedef enum {
+    SIMLAB_FACE_VELOCITY = 0,
+    SIMLAB_FACE_OPEN = 1,
+    SIMLAB_FACE_PERIODIC = 2
+} simlab_face_bc;
+
+/* `LatticeSetup`, lattice.rs:45-76; field meaning and units as there. */
+typedef struct simlab_lattice_setup {
+    uint32_t dims[3];
+    int32_t faces[6];            /* -x, +x, -y, +y, -z, +z (simlab_face_bc) */
+    double nu;                   /* lattice kinematic viscosity             */
+    double smagorinsky;          /* C_s (0 = no turbulence model)           */
+    double u_inf[3];
+    double u_init[3];
+    double body_force[3];        /* uniform acceleration                    */
+    uint32_t sponge[6];          /* viscosity sponge width per face, cells  */
+    double sponge_tau;
+    double sponge_sigma;
+    int32_t absorb_velocity;
+    int32_t total_pressure_inlet;
+    int32_t storage;             /* simlab_storage                          */
+    int32_t debug_dump;          /* keep the buffer `macros()` reads        */
+    double center[3];            /* moment reference point / spin centre    */
+    int32_t wall_model;          /* Spalding wall model in wall cells       */
+    int32_t reserved0;
+} simlab_lattice_setup;
+
+/* `WallCell`, voxel.rs:328-335: a wall-adjacent fluid cell, the bit mask of
+ * its cut links (bit i = direction i) and the Bouzidi q per direction. */
+typedef struct simlab_wall_cell {
+    uint32_t cell;
+    uint32_t mask;
+    float q[27];
+} simlab_wall_cell;
+
+enum { SIMLAB_GEOM_NONE = 0, SIMLAB_GEOM_STATIC = 1, SIMLAB_GEOM_SPIN = 2 };
+
+/* `Geometry`, lattice.rs:87-92. Static: `StaticVoxels` (voxel.rs:337-344):
+ * solid cell indices and wall cells (sorted by cell). Spin: `SpinVoxels`
+ * (voxel.rs:492-505) plus the swept box in lattice cells. */
+typedef struct simlab_geometry {
+    int32_t kind;                /* SIMLAB_GEOM_*                           */
+    int32_t reserved0;
+    uint64_t nsolid;
+    const uint32_t *solid;
+    uint64_t nwall;
+    const simlab_wall_cell *walls;
+    /* spin */
+    double occ_lo[3];
+    double occ_h;
+    uint32_t occ_dims[3];
+    uint32_t reserved1;
+    uint64_t nocc_words;
+    const uint32_t *occ_bits;
+    uint64_t ntri;
+    const float *tris;           /* ntri x 12: v0.xyz,0, e1.xyz,0, e2.xyz,0 */
+    int32_t box_lo[3];
+    uint32_t box_n[3];
+} simlab_geometry;
+
+/* `SliceDef`, shader.rs:77-84. */
+typedef struct simlab_slice_def {
+    int32_t origin[3];
+    int32_t u[3];
+    int32_t v[3];
+    uint32_t nu;
+    uint32_t nv;
+} simlab_slice_def;
+
+/* `Probes`, lattice.rs:94-102. verts: nvert x 6 doubles (body-frame
+ * position relative to the centre, then the unit normal). */
+typedef struct simlab_probes {
+    uint64_t nvert;
+    const double *verts;
+    uint64_t nslice;
+    const simlab_slice_def *slices;
+    double vertex_offset;
+} simlab_probes;
+
+/* `StepIn`, lattice.rs:104-121. rot: row-major 3x3 whose columns are the
+ * body axes in lattice coordinates; omega in rad per step. */
+typedef struct simlab_step_in {
+    double rot[3][3];
+    double rot_prev[3][3];
+    double omega[3];
+    int32_t sample;
+    int32_t reserved0;
+} simlab_step_in;
+
+/* Public fields of `Lattice` (lattice.rs:148-170). */
+typedef struct simlab_lattice_info {
+    uint64_t steps_done;
+    uint64_t cells;
+    uint64_t memory_bytes;
+    uint32_t dims[3];
+    uint32_t nrec;
+    uint32_t nface;
+    uint32_t npix;
+    uint32_t nvert;
+    int32_t storage;             /* effective storage                       */
+    int32_t moving;
+    double tau0;
+} simlab_lattice_info;
+
+typedef struct simlab_lattice simlab_lattice;            /* opaque */
+typedef struct simlab_static_voxels simlab_static_voxels; /* opaque */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *simlab_cfd_last_error(void);
+
+/* `gpu_info` (gpu.rs:56-63): "name · HIP · gfx · GB max buffer". */
+int simlab_cfd_gpu_info(char *buf, size_t len);
+
+/* `Lattice::max_cells` (lattice.rs:223-227): most cells a lattice with this
+ * storage can hold by the buffer layout alone (memory budgets are the
+ * caller's). */
+uint64_t simlab_cfd_max_cells(int32_t storage);
+
+/* `Lattice::new` (lattice.rs:229-673). probes may be NULL. */
+int simlab_cfd_lattice_new(const simlab_lattice_setup *setup,
+                           const simlab_geometry *geom,
+                           const simlab_probes *probes,
+                           simlab_lattice **out);
+
+/* Drop a lattice and its device buffers (Rust `Drop`). NULL is a no-op. */
+void simlab_cfd_lattice_free(simlab_lattice *lat);
+
+/* Public fields of the lattice. */
+int simlab_cfd_lattice_info(const simlab_lattice *lat, simlab_lattice_info *out);
+
+/* `Lattice::run` (lattice.rs:699-753): runs nsteps steps and writes the
+ * force and moment on the geometry after each step (nsteps x 6 doubles:
+ * Fx Fy Fz Tx Ty Tz, lattice units, moment about `center`). The sum over
+ * the wall records is accumulated in FP64 on the device. ft_out may be NULL. */
+int simlab_cfd_lattice_run(simlab_lattice *lat, const simlab_step_in *steps,
+                           size_t nsteps, double *ft_out);
+
+/* `run` with nsteps copies of `StepIn::still(false)` (lattice.rs:118-120),
+ * the form every test and benchmark of the reference uses. */
+int simlab_cfd_lattice_run_still(simlab_lattice *lat, size_t nsteps,
+                                 double *ft_out);
+
+/* `Lattice::macros` (lattice.rs:818-835): (rho, ux, uy, uz) of every cell,
+ * NaN where solid or on a face; needs debug_dump. out: cells x 4 floats. */
+int simlab_cfd_lattice_macros(simlab_lattice *lat, float *out);
+
+/* `Lattice::records` (lattice.rs:837-841): max(nrec,1) x 40 words. */
+int simlab_cfd_lattice_records(simlab_lattice *lat, uint32_t *out);
+
+/* `Lattice::reset_samples` (lattice.rs:785-791). */
+int simlab_cfd_lattice_reset_samples(simlab_lattice *lat);
+
+/* `Lattice::vertex_means` (lattice.rs:793-801): nvert floats. */
+int simlab_cfd_lattice_vertex_means(simlab_lattice *lat, float *out);
+
+/* `Lattice::slice_means` (lattice.rs:803-816): npix x 4 floats. */
+int simlab_cfd_lattice_slice_means(simlab_lattice *lat, float *out);
+
+/* Total fluid mass sum(rho) over the cells that collide (fluid, not face,
+ * not solid), accumulated in FP64 on the device. New: the reference has no
+ * mass check; it is the north star's global conservation check. */
+int simlab_cfd_lattice_mass(simlab_lattice *lat, double *mass);
+
+/* Raw population slots in storage order (19 x cells elements of the storage
+ * type: float, uint16 half/bf16 bits, or double), for checkpoints and tests.
+ * `bytes` must equal 19 * cells * element size. */
+int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes);
+int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t bytes);
+
+/* Device timing of the last `run` call: total milliseconds of its steps and
+ * the summed duration and count of the stream-collide launches, measured with
+ * HIP events on the lattice's stream (timing must have been enabled). */
+int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
+int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
+                              double *main_ms, uint64_t *main_launches);
+
+/* `voxelize_static` (voxel.rs:348-352, 363-490) on the host CPU: solid cells
+ * and wall links of a triangle surface given in lattice coordinates (cell
+ * centres at integer points). verts: nverts x 3 doubles; tris: ntris x 3. */
+int simlab_cfd_voxelize_static(const double *verts, uint64_t nverts,
+                               const uint32_t *tris, uint64_t ntris,
+                               const uint64_t dims[3],
+                               simlab_static_voxels **out);
+/* Views into a voxelization (valid until it is freed); lo/hi: examined box. */
+int simlab_cfd_voxels_view(const simlab_static_voxels *v, uint64_t *nsolid,
+                           const uint32_t **solid, uint64_t *nwall,
+                           const simlab_wall_cell **walls, uint64_t lo[3],
+                           uint64_t hi[3]);
+void simlab_cfd_voxels_free(simlab_static_voxels *v);
+
+/* `inside_grid` (voxel.rs:73-95): inside flags (0/1) for the grid points
+ * origin + h * (i, j, k), i fastest. out: dims[0]*dims[1]*dims[2] bytes. */
+int simlab_cfd_inside_grid(const double *verts, uint64_t nverts,
+                           const uint32_t *tris, uint64_t ntris,
+                           const double origin[3], double h,
+                           const uint64_t dims[3], uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMLAB_CFD_H */

@@ -1,0 +1,229 @@
+"""ORACLE — test infrastructure only (tests/, the smoke check and bench.py's
+CPU-baseline leg). ctypes wrapper of liboracle.so, the CPU restatement of
+the reference's D3Q19 lattice (see lbm_oracle.cpp). It takes the same setup
+and geometry structures as the product library so both run on identical
+inputs."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+import simlab_amd as sa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+# oracle storages: the public ones plus FP16 with the reference's per-store hash
+F32, F16, BF16, F64, F16REF = 0, 1, 2, 3, 4
+_POP = {F32: np.float32, F16: np.uint16, BF16: np.uint16, F64: np.float64, F16REF: np.uint16}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        V, P = C.c_void_p, C.c_void_p
+        L.oracle_new.restype = V
+        L.oracle_new.argtypes = [C.POINTER(sa._Setup), C.POINTER(sa._Geometry), C.POINTER(sa._Probes), C.c_int, C.c_int]
+        L.oracle_free.argtypes = [V]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_run.argtypes = [V, C.POINTER(sa._StepIn), C.c_uint64, P]
+        L.oracle_run_still.argtypes = [V, C.c_uint64, P]
+        L.oracle_steps_done.restype = C.c_uint64
+        L.oracle_steps_done.argtypes = [V]
+        for n in ("oracle_nrec", "oracle_nface"):
+            getattr(L, n).restype = C.c_uint32
+            getattr(L, n).argtypes = [V]
+        for n in ("oracle_macros", "oracle_macros_f64", "oracle_records", "oracle_get_pops", "oracle_set_pops",
+                  "oracle_get_flags", "oracle_get_faces", "oracle_vertex_means", "oracle_slice_means"):
+            getattr(L, n).argtypes = [V, P]
+        L.oracle_reset_samples.argtypes = [V]
+        L.oracle_mass.restype = C.c_double
+        L.oracle_mass.argtypes = [V]
+        L.oracle_sround_ref.restype = C.c_float
+        L.oracle_sround_ref.argtypes = [C.c_float, C.c_uint32, C.c_uint32]
+        for n in ("oracle_f16_bits_host", "oracle_f16_rne", "oracle_f16_rtz"):
+            getattr(L, n).restype = C.c_uint16
+            getattr(L, n).argtypes = [C.c_float]
+        L.oracle_half_to_float.restype = C.c_float
+        L.oracle_half_to_float.argtypes = [C.c_uint16]
+        L.oracle_cell_hash.restype = C.c_uint32
+        L.oracle_cell_hash.argtypes = [C.c_uint32, C.c_uint32]
+        L.oracle_dither16.argtypes = [C.c_uint32, C.c_uint32, P]
+        L.oracle_enc_f16_sr.restype = C.c_uint16
+        L.oracle_enc_f16_sr.argtypes = [C.c_float, C.c_uint16]
+        L.oracle_enc_bf16_sr.restype = C.c_uint16
+        L.oracle_enc_bf16_sr.argtypes = [C.c_float, C.c_uint16]
+        L.oracle_aa_maps.argtypes = [P, C.c_int, P, P]
+        L.oracle_collide_f64.argtypes = [P, C.c_double, C.c_double, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleLattice:
+    """CPU restatement of `Lattice` (lattice.rs:220-841)."""
+
+    def __init__(self, setup: sa.LatticeSetup, geometry=None, probes=None, store: int = F32, threads: int = 0):
+        L = lib()
+        self._lib = L
+        self.setup = setup
+        self.store = store
+        cs = setup.to_c()
+        g, keep_g = sa.geometry_to_c(geometry)
+        p, keep_p = sa.probes_to_c(probes)
+        nthreads = threads or min(os.cpu_count() or 1, 16)
+        self._h = L.oracle_new(C.byref(cs), C.byref(g), C.byref(p) if p is not None else None, int(store), nthreads)
+        del keep_g, keep_p
+        if not self._h:
+            raise sa.SimlabError(L.oracle_last_error().decode())
+        self.nvert = len(probes.verts) if probes else 0
+        self.npix = sum(s.nu * s.nv for s in probes.slices) if probes else 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.oracle_free(h)
+            self._h = None
+
+    @property
+    def steps_done(self):
+        return int(self._lib.oracle_steps_done(self._h))
+
+    @property
+    def nrec(self):
+        return int(self._lib.oracle_nrec(self._h))
+
+    @property
+    def nface(self):
+        return int(self._lib.oracle_nface(self._h))
+
+    def run(self, steps):
+        out = np.zeros((len(steps), 6), np.float64)
+        arr = sa.steps_to_c(steps)
+        self._lib.oracle_run(self._h, arr, len(steps), _ptr(out))
+        return out
+
+    def run_still(self, n):
+        out = np.zeros((int(n), 6), np.float64)
+        self._lib.oracle_run_still(self._h, int(n), _ptr(out))
+        return out
+
+    def macros(self):
+        out = np.zeros((self.setup.cells(), 4), np.float32)
+        self._lib.oracle_macros(self._h, _ptr(out))
+        return out
+
+    def macros_f64(self):
+        out = np.zeros((self.setup.cells(), 4), np.float64)
+        self._lib.oracle_macros_f64(self._h, _ptr(out))
+        return out
+
+    def records(self):
+        out = np.zeros((max(self.nrec, 1), 40), np.uint32)
+        self._lib.oracle_records(self._h, _ptr(out))
+        return out
+
+    def get_pops(self):
+        out = np.zeros(19 * self.setup.cells(), _POP[self.store])
+        self._lib.oracle_get_pops(self._h, _ptr(out))
+        return out
+
+    def set_pops(self, a):
+        a = np.ascontiguousarray(a, dtype=_POP[self.store])
+        self._lib.oracle_set_pops(self._h, _ptr(a))
+
+    def flags(self):
+        out = np.zeros(self.setup.cells(), np.uint32)
+        self._lib.oracle_get_flags(self._h, _ptr(out))
+        return out
+
+    def faces(self):
+        out = np.zeros(max(self.nface, 1), np.uint32)
+        self._lib.oracle_get_faces(self._h, _ptr(out))
+        return out[: self.nface]
+
+    def mass(self):
+        return float(self._lib.oracle_mass(self._h))
+
+    def reset_samples(self):
+        self._lib.oracle_reset_samples(self._h)
+
+    def vertex_means(self):
+        out = np.zeros(self.nvert, np.float32)
+        self._lib.oracle_vertex_means(self._h, _ptr(out))
+        return out
+
+    def slice_means(self):
+        out = np.zeros((self.npix, 4), np.float32)
+        self._lib.oracle_slice_means(self._h, _ptr(out))
+        return out
+
+
+def sround_ref(x, key, seed):
+    return float(lib().oracle_sround_ref(float(x), int(key), int(seed)))
+
+
+def f16_bits_host(x):
+    return int(lib().oracle_f16_bits_host(float(x)))
+
+
+def f16_rne(x):
+    return int(lib().oracle_f16_rne(float(x)))
+
+
+def f16_rtz(x):
+    return int(lib().oracle_f16_rtz(float(x)))
+
+
+def half_to_float(h):
+    return float(lib().oracle_half_to_float(int(h)))
+
+
+def cell_hash(n, seed):
+    return int(lib().oracle_cell_hash(int(n), int(seed)))
+
+
+def dither16(n, seed):
+    out = np.zeros(19, np.uint16)
+    lib().oracle_dither16(int(n), int(seed), _ptr(out))
+    return out
+
+
+def enc_f16_sr(x, d):
+    return int(lib().oracle_enc_f16_sr(float(x), int(d)))
+
+
+def enc_bf16_sr(x, d):
+    return int(lib().oracle_enc_bf16_sr(float(x), int(d)))
+
+
+def aa_maps(dims, p):
+    dims = np.asarray(dims, np.uint32)
+    n = int(np.prod(dims.astype(np.int64)))
+    ld = np.zeros((n, 19), np.uint64)
+    st = np.zeros((n, 19), np.uint64)
+    lib().oracle_aa_maps(_ptr(dims), int(p), _ptr(ld), _ptr(st))
+    return ld, st
+
+
+def collide_f64(fin, tau0, cs=0.0, grav=(0.0, 0.0, 0.0)):
+    fin = np.ascontiguousarray(fin, np.float64)
+    g = np.ascontiguousarray(grav, np.float64)
+    out = np.zeros(19, np.float64)
+    lib().oracle_collide_f64(_ptr(fin), float(tau0), float(cs), _ptr(g), _ptr(out))
+    return out

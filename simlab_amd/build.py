@@ -1,0 +1,65 @@
+"""Builds libsimlab_cfd.so (gfx950 kernels + host library) in-tree.
+
+hipcc compiles the kernels and the HIP host code; g++ compiles the OpenMP
+voxelizer. The shared library lands next to this file; objects go to
+simlab_amd/_build/. Every build recompiles (the caller decides when).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libsimlab_cfd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+# packed-FP32 instructions are disabled for every HIP translation unit
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=fast",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function",
+             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ...")
+    elif verbose and r.stdout.strip():
+        # hipcc's host pass warns about the packed-fp32 feature; keep it short
+        lines = [l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l]
+        if lines:
+            print("\n".join(lines[-20:]))
+
+
+def build(verbose=False, save_temps=False):
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    for src in ("kernels.hip", "lattice.cpp"):
+        obj = os.path.join(OUT, src + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        if save_temps:
+            cmd.insert(1, "--save-temps")
+        _run(cmd, verbose)
+        objs.append(obj)
+    for src in ("voxel.cpp",):
+        obj = os.path.join(OUT, src + ".o")
+        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True, save_temps="--save-temps" in sys.argv)
+    print(LIB)

@@ -1,0 +1,97 @@
+// Interface between the host lattice (lattice.cpp) and the gfx950 kernels
+// (kernels.hip). Plain structs passed by value as kernel arguments.
+#pragma once
+
+#include "../../include/simlab_cfd.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace simlab {
+
+constexpr uint32_t KMAX = 512;   // steps per device-to-host force copy (lattice.rs:134)
+constexpr uint32_t HIST_STRIDE = 8;   // doubles per step in the force history
+
+// Device view of one lattice. Population slot of direction i and cell n is
+// element i * ncell + n of `pops` (SoA, x fastest, as shader.rs:147-152).
+// Wall records are SoA too: field j of record r is element j * nrec + r.
+struct DevLattice {
+    uint32_t nx, ny, nz;
+    uint32_t ncell;
+    uint32_t per[3];
+    int32_t storage;             // simlab_storage
+
+    void *pops;
+    uint32_t *flags;             // bits 0-1 type (0 fluid, 1 solid, 2 velocity face, 3 open face), bits 2.. record+1
+
+    uint32_t nrec;
+    uint32_t *rmask;             // [nrec]
+    uint32_t *rcell;             // [nrec]
+    uint32_t *rq;                // [9][nrec] f16 pairs (dir 2k+1 low, 2k+2 high)
+    void *rfpost;                // [18][nrec] post-collision f~ of dirs 1..18 (real)
+    void *rmac;                  // [4][nrec] rho, ux, uy, uz (real)
+    void *rft;                   // [6][nrec] F, T (real)
+
+    uint32_t nface;
+    uint32_t *faces;
+
+    double *hist;                // [KMAX][HIST_STRIDE]
+    double *partials;            // [wall_blocks][8]
+    uint32_t *counter;           // last-block detection, returns to 0 after each step
+
+    // physics (lattice units)
+    double tau0, smag2, sponge_tau, sigma;
+    double u_inf[3], u_init[3], grav[3], center[3];
+    double sp_lo[3], sp_hi[3];
+    int32_t abs_u, total_p, wall_model, moving, forcing, sponge;
+
+    // launch shape of the stream-collide kernel
+    uint32_t bx;                 // threads along x per block (multiple of 64)
+    uint32_t by;                 // x-rows per block
+    uint32_t xblocks;            // ceil(nx / bx)
+    uint32_t yblocks;            // ceil(ny / by)
+    uint32_t wall_blocks;        // blocks that process the records (>= 1)
+    uint32_t wall_rows;          // grid rows taken by the wall blocks
+    uint32_t xcd_remap;          // remap block ids so neighbouring tiles share an XCD
+
+    // moving geometry (swept box + body-frame occupancy)
+    int32_t box_lo[3];
+    uint32_t box_n[3];
+    float occ_lo[3];
+    float occ_invh;
+    int32_t occ_n[3];
+    const uint32_t *occ;
+    uint32_t ntri;
+    const float4 *tris;          // ntri x 3 (v0, e1, e2)
+    uint32_t *qscr;              // [nrec * 9 * 3]
+
+    // probes
+    uint32_t nvert;
+    const float4 *verts;         // nvert x 2 (position, normal)
+    float vertex_offset;
+    float *vacc;                 // nvert x 2
+    uint32_t npix;
+    const int4 *slicedef;
+    float *sacc;                 // npix x 5
+};
+
+// Per-step uniform (lattice.rs:123-132): body rotation now / before (columns
+// = body axes), angular velocity, stochastic-rounding seed, history slot.
+struct StepArgs {
+    float r[3][4];
+    float q[3][4];
+    float omega[4];
+    uint32_t seed;
+    uint32_t slot;
+};
+
+hipError_t launch_init(const DevLattice &L, hipStream_t s);
+hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
+hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
+hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
+hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
+
+}  // namespace simlab

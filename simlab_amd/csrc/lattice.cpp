@@ -1,0 +1,746 @@
+// Host side of the MI355X lattice: setup validation, face list, flags and
+// wall records (crates/cfd/src/lattice.rs:229-526), device buffers, the
+// per-step launch sequence (lattice.rs:699-753) and the read-backs
+// (lattice.rs:755-841), behind the C ABI of include/simlab_cfd.h.
+
+#include "kernels.h"
+#include "simlab_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace simlab {
+
+namespace {
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        hipGetLastError();   // clear the sticky error of failed launches
+        throw HipError(e == hipErrorOutOfMemory ? SIMLAB_ERR_OOM : SIMLAB_ERR_GPU,
+                       std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+size_t elem_bytes(int storage) {
+    switch (storage) {
+        case SIMLAB_STORAGE_FP32: return 4;
+        case SIMLAB_STORAGE_FP64: return 8;
+        default: return 2;
+    }
+}
+size_t real_bytes(int storage) { return storage == SIMLAB_STORAGE_FP64 ? 8 : 4; }
+
+uint16_t f16_bits_host(float x) {   // lattice.rs:172-189
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    uint16_t sign = (uint16_t)((b >> 16) & 0x8000);
+    int exp = (int)((b >> 23) & 0xff) - 127 + 15;
+    uint32_t man = b & 0x7fffff;
+    if (exp <= 0) return sign;
+    if (exp >= 31) return sign | 0x7c00;
+    uint32_t h = sign | ((uint32_t)exp << 10) | (man >> 13);
+    if (man & 0x1000) h += 1;
+    return (uint16_t)h;
+}
+
+float half_to_float(uint16_t h) {
+    uint32_t s = (uint32_t)(h & 0x8000) << 16, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    float v;
+    if (e == 0) {
+        v = std::ldexp((float)m, -24);
+        return s ? -v : v;
+    }
+    uint32_t bits = e == 31 ? (s | 0x7f800000u | (m << 13)) : (s | ((e + 112) << 23) | (m << 13));
+    std::memcpy(&v, &bits, 4);
+    return v;
+}
+
+// Device buffers owned by a lattice; freed in the destructor.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+    void alloc(size_t n, size_t &total) {
+        bytes = std::max<size_t>(n, 16);
+        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        total += bytes;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace simlab
+
+using namespace simlab;
+
+struct simlab_lattice {
+    int device = 0;
+    simlab_lattice_setup setup{};
+    int storage = 0;
+    uint64_t ncell = 0;
+    uint64_t steps_done = 0;
+    uint32_t nrec = 0, nface = 0, npix = 0, nvert = 0;
+    uint64_t memory_bytes = 0;
+    bool moving = false;
+    hipStream_t stream = nullptr;
+    DevLattice L{};
+    DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, partials, counter, dump, mass_part,
+        mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
+    double *hist_host = nullptr;   // pinned, KMAX x HIST_STRIDE
+    bool timing = false;
+    std::vector<hipEvent_t> ev;    // 2 per step of a chunk + 2 around the chunk
+    double last_step_ms = 0.0, last_main_ms = 0.0;
+    uint64_t last_main_launches = 0;
+
+    ~simlab_lattice() {
+        if (stream) hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        if (hist_host) hipHostFree(hist_host);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+const char *validate(const simlab_lattice_setup *s, std::string &msg) {
+    // lattice.rs:233-244
+    uint64_t n = (uint64_t)s->dims[0] * s->dims[1] * s->dims[2];
+    if (n == 0 || n >= 0xffffffffull) {
+        msg = "grid of " + std::to_string(n) + " cells is out of range";
+        return msg.c_str();
+    }
+    for (int k = 0; k < 3; ++k)
+        if (s->dims[k] < 3) return "grid must be at least 3 cells in every direction";
+    for (int a = 0; a < 3; ++a)
+        if ((s->faces[2 * a] == SIMLAB_FACE_PERIODIC) != (s->faces[2 * a + 1] == SIMLAB_FACE_PERIODIC))
+            return "periodic faces must come in pairs";
+    for (int f = 0; f < 6; ++f)
+        if (s->faces[f] < 0 || s->faces[f] > 2) return "unknown face boundary condition";
+    if (s->storage < 0 || s->storage > 3) return "unknown population storage";
+    return nullptr;
+}
+
+// Launch shape: whole x-rows per block. bx = nx rounded up to a wave (at most
+// 1024 lanes), by rows so a block has about `target` threads.
+void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) {
+    uint32_t bx = ((L.nx + 63) / 64) * 64;
+    if (bx > 1024) bx = 1024;
+    uint32_t by = by_override ? by_override : std::max<uint32_t>(1, target_threads / bx);
+    by = std::max<uint32_t>(1, std::min<uint32_t>(by, std::min<uint32_t>(1024 / bx, L.ny)));
+    L.bx = bx;
+    L.by = by;
+    L.xblocks = (L.nx + bx - 1) / bx;
+    L.yblocks = (L.ny + by - 1) / by;
+    const uint32_t bs = bx * by;
+    L.wall_blocks = std::max<uint32_t>(1, (L.nrec + bs - 1) / bs);
+    const uint32_t gx = L.xblocks * L.yblocks;
+    L.wall_rows = (L.wall_blocks + gx - 1) / gx;
+}
+
+uint32_t default_block_threads() {
+    const char *e = std::getenv("SIMLAB_CFD_BLOCK");
+    if (e) {
+        long v = std::strtol(e, nullptr, 10);
+        if (v >= 64 && v <= 1024) return (uint32_t)v;
+    }
+    return 256;
+}
+
+void pack_step(const simlab_step_in &s, uint64_t step, uint32_t slot, StepArgs &a) {
+    // lattice.rs:675-686: columns of the rotations as f32; seed per step
+    for (int k = 0; k < 3; ++k) {
+        for (int r = 0; r < 3; ++r) {
+            a.r[k][r] = (float)s.rot[r][k];
+            a.q[k][r] = (float)s.rot_prev[r][k];
+        }
+        a.r[k][3] = 0.0f;
+        a.q[k][3] = 0.0f;
+        a.omega[k] = (float)s.omega[k];
+    }
+    a.omega[3] = 0.0f;
+    a.seed = (uint32_t)step * 2654435761u;
+    a.slot = slot;
+}
+
+int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const simlab_probes *pr, simlab_lattice **out) {
+    if (!su || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::string msg;
+    if (const char *m = validate(su, msg)) return fail(SIMLAB_ERR_INVALID, m);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        hipGetLastError();
+        return fail(SIMLAB_ERR_GPU, "no usable GPU: no HIP device found");
+    }
+    auto lat = std::make_unique<simlab_lattice>();
+    hip_check(hipGetDevice(&lat->device), "hipGetDevice");
+    lat->setup = *su;
+    lat->storage = su->storage;
+    const uint32_t nx = su->dims[0], ny = su->dims[1], nz = su->dims[2];
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    lat->ncell = n;
+    const int geom_kind = g ? g->kind : SIMLAB_GEOM_NONE;
+    if (geom_kind == SIMLAB_GEOM_SPIN) {
+        for (int k = 0; k < 3; ++k)
+            if (g->box_lo[k] < 1 || (int64_t)g->box_lo[k] + g->box_n[k] > (int64_t)su->dims[k] - 1)
+                return fail(SIMLAB_ERR_INVALID, "the spinning part's swept box leaves the domain");
+        return fail(SIMLAB_ERR_INVALID, "spinning geometry is not available in this build");
+    }
+    if (geom_kind == SIMLAB_GEOM_STATIC) {
+        for (uint64_t r = 0; r < g->nwall; ++r)
+            if (g->walls[r].cell >= n) return fail(SIMLAB_ERR_INVALID, "wall cell outside the grid");
+        for (uint64_t r = 0; r < g->nsolid; ++r)
+            if (g->solid[r] >= n) return fail(SIMLAB_ERR_INVALID, "solid cell outside the grid");
+    }
+
+    // face cells of non-periodic axes (lattice.rs:283-324)
+    auto face_type = [&](uint32_t x, uint32_t y, uint32_t z) -> int {
+        const bool on[6] = {x == 0, x == nx - 1, y == 0, y == ny - 1, z == 0, z == nz - 1};
+        int ty = -1;
+        for (int f = 0; f < 6; ++f)
+            if (on[f] && su->faces[f] != SIMLAB_FACE_PERIODIC) {
+                const int t = su->faces[f] == SIMLAB_FACE_VELOCITY ? 2 : 3;
+                ty = (ty == 2) ? 2 : t;
+            }
+        return ty;
+    };
+    std::vector<uint32_t> faces;
+    for (uint32_t z = 0; z < nz; ++z)
+        for (uint32_t y = 0; y < ny; ++y) {
+            const bool edge = y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
+            if (edge) {
+                for (uint32_t x = 0; x < nx; ++x)
+                    if (face_type(x, y, z) >= 0) faces.push_back((z * ny + y) * nx + x);
+            } else {
+                for (uint32_t x : {0u, nx - 1})
+                    if (face_type(x, y, z) >= 0) faces.push_back((z * ny + y) * nx + x);
+            }
+        }
+    lat->nface = (uint32_t)faces.size();
+
+    // wall records, SoA (record_words, lattice.rs:202-218)
+    const uint32_t nrec = geom_kind == SIMLAB_GEOM_STATIC ? (uint32_t)g->nwall : 0u;
+    lat->nrec = nrec;
+    const size_t rb = real_bytes(su->storage);
+    std::vector<uint32_t> h_mask(nrec), h_cell(nrec), h_q((size_t)9 * nrec);
+    std::vector<unsigned char> h_fpost((size_t)18 * nrec * rb), h_mac((size_t)4 * nrec * rb), h_ft((size_t)6 * nrec * rb, 0);
+    {
+        const double *ui = su->u_init;
+        const double usq = 1.5 * (ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
+        float fe[19];   // feq_shifted (lattice.rs:191-200)
+        for (int i = 0; i < 19; ++i) {
+            const double cu = CV[i][0] * ui[0] + CV[i][1] * ui[1] + CV[i][2] * ui[2];
+            fe[i] = (float)(weight(i) * (3.0 * cu + 4.5 * cu * cu - usq));
+        }
+        auto put = [&](std::vector<unsigned char> &v, size_t idx, float x) {
+            if (rb == 8) {
+                double d = (double)x;
+                std::memcpy(&v[idx * 8], &d, 8);
+            } else {
+                std::memcpy(&v[idx * 4], &x, 4);
+            }
+        };
+        for (uint32_t r = 0; r < nrec; ++r) {
+            const simlab_wall_cell &w = g->walls[r];
+            h_mask[r] = w.mask;
+            h_cell[r] = w.cell;
+            for (int k = 0; k < 9; ++k)
+                h_q[(size_t)k * nrec + r] = (uint32_t)f16_bits_host(w.q[2 * k + 1]) | ((uint32_t)f16_bits_host(w.q[2 * k + 2]) << 16);
+            for (int i = 1; i < 19; ++i) put(h_fpost, (size_t)(i - 1) * nrec + r, fe[i]);
+            put(h_mac, r, 1.0f);
+            for (int k = 0; k < 3; ++k) put(h_mac, (size_t)(k + 1) * nrec + r, (float)ui[k]);
+        }
+    }
+    // flags: face type, then solid, then record index + 1 (lattice.rs:489-526)
+    std::vector<uint32_t> h_flags(n, 0);
+    for (uint32_t f : faces) {
+        uint32_t z = f / (nx * ny), r = f - z * nx * ny, y = r / nx, x = r - y * nx;
+        h_flags[f] = (uint32_t)face_type(x, y, z);
+    }
+    if (geom_kind == SIMLAB_GEOM_STATIC) {
+        for (uint64_t k = 0; k < g->nsolid; ++k)
+            if (h_flags[g->solid[k]] == 0) h_flags[g->solid[k]] = 1;
+        for (uint32_t r = 0; r < nrec; ++r) {
+            const uint32_t c = g->walls[r].cell;
+            if ((h_flags[c] & 3u) == 0) h_flags[c] = (r + 1) << 2;
+        }
+    }
+
+    // probes
+    std::vector<float4> h_verts;
+    std::vector<int4> h_slices;
+    if (pr) {
+        for (uint64_t v = 0; v < pr->nvert; ++v) {
+            const double *d = &pr->verts[6 * v];
+            h_verts.push_back(make_float4((float)d[0], (float)d[1], (float)d[2], 0.0f));
+            h_verts.push_back(make_float4((float)d[3], (float)d[4], (float)d[5], 0.0f));
+        }
+        h_slices.push_back(make_int4((int)pr->nslice, 0, 0, 0));
+        for (uint64_t s = 0; s < pr->nslice; ++s) {
+            const simlab_slice_def &d = pr->slices[s];
+            for (int k = 0; k < 3; ++k) {
+                const int64_t lo = (int64_t)d.origin[k], hi = lo + (int64_t)(d.nu ? d.nu - 1 : 0) * d.u[k] + (int64_t)(d.nv ? d.nv - 1 : 0) * d.v[k];
+                if (std::min(lo, hi) < 0 || std::max(lo, hi) >= (int64_t)su->dims[k]) return fail(SIMLAB_ERR_INVALID, "slice leaves the grid");
+            }
+            h_slices.push_back(make_int4(d.origin[0], d.origin[1], d.origin[2], (int)d.nu));
+            h_slices.push_back(make_int4(d.u[0], d.u[1], d.u[2], (int)d.nv));
+            h_slices.push_back(make_int4(d.v[0], d.v[1], d.v[2], 0));
+            lat->npix += d.nu * d.nv;
+        }
+        lat->nvert = (uint32_t)pr->nvert;
+    }
+
+    // device buffers
+    size_t mem = 0;
+    const size_t eb = elem_bytes(su->storage);
+    try {
+        lat->pops.alloc((size_t)19 * n * eb, mem);
+        lat->flags.alloc(n * 4, mem);
+        lat->rmask.alloc((size_t)nrec * 4, mem);
+        lat->rcell.alloc((size_t)nrec * 4, mem);
+        lat->rq.alloc((size_t)9 * nrec * 4, mem);
+        lat->rfpost.alloc((size_t)18 * nrec * rb, mem);
+        lat->rmac.alloc((size_t)4 * nrec * rb, mem);
+        lat->rft.alloc((size_t)6 * nrec * rb, mem);
+        lat->faces.alloc((size_t)lat->nface * 4, mem);
+        lat->hist.alloc((size_t)KMAX * HIST_STRIDE * 8, mem);
+        lat->counter.alloc(256, mem);
+        lat->dump.alloc(su->debug_dump ? n * 16 : 16, mem);
+        lat->mass_part.alloc(1024 * 8, mem);
+        lat->mass_out.alloc(8, mem);
+        lat->verts.alloc(h_verts.size() * sizeof(float4), mem);
+        lat->vacc.alloc((size_t)lat->nvert * 8, mem);
+        lat->slicedef.alloc(std::max<size_t>(1, h_slices.size()) * sizeof(int4), mem);
+        lat->sacc.alloc((size_t)lat->npix * 20, mem);
+    } catch (const HipError &e) {
+        if (e.code == SIMLAB_ERR_OOM) {
+            char b[160];
+            std::snprintf(b, sizeof b, "not enough GPU memory for this grid (%.1f GB): %s", (double)(19 * n * eb + n * 4) / 1e9, e.what());
+            return fail(SIMLAB_ERR_OOM, b);
+        }
+        throw;
+    }
+    hip_check(hipStreamCreateWithFlags(&lat->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipHostMalloc((void **)&lat->hist_host, (size_t)KMAX * HIST_STRIDE * 8, hipHostMallocDefault), "hipHostMalloc");
+    hipStream_t st = lat->stream;
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), "upload");
+    };
+    up(lat->flags, h_flags.data(), n * 4);
+    up(lat->rmask, h_mask.data(), (size_t)nrec * 4);
+    up(lat->rcell, h_cell.data(), (size_t)nrec * 4);
+    up(lat->rq, h_q.data(), h_q.size() * 4);
+    up(lat->rfpost, h_fpost.data(), h_fpost.size());
+    up(lat->rmac, h_mac.data(), h_mac.size());
+    up(lat->rft, h_ft.data(), h_ft.size());
+    up(lat->faces, faces.data(), faces.size() * 4);
+    up(lat->verts, h_verts.data(), h_verts.size() * sizeof(float4));
+    up(lat->slicedef, h_slices.data(), h_slices.size() * sizeof(int4));
+    hip_check(hipMemsetAsync(lat->counter.p, 0, 256, st), "memset");
+    hip_check(hipMemsetAsync(lat->hist.p, 0, lat->hist.bytes, st), "memset");
+    hip_check(hipMemsetAsync(lat->vacc.p, 0, lat->vacc.bytes, st), "memset");
+    hip_check(hipMemsetAsync(lat->sacc.p, 0, lat->sacc.bytes, st), "memset");
+    lat->memory_bytes = mem;
+
+    DevLattice &L = lat->L;
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.ncell = (uint32_t)n;
+    for (int a = 0; a < 3; ++a) L.per[a] = su->faces[2 * a] == SIMLAB_FACE_PERIODIC ? 1u : 0u;
+    L.storage = su->storage;
+    L.pops = lat->pops.p;
+    L.flags = lat->flags.as<uint32_t>();
+    L.nrec = nrec;
+    L.rmask = lat->rmask.as<uint32_t>();
+    L.rcell = lat->rcell.as<uint32_t>();
+    L.rq = lat->rq.as<uint32_t>();
+    L.rfpost = lat->rfpost.p;
+    L.rmac = lat->rmac.p;
+    L.rft = lat->rft.p;
+    L.nface = lat->nface;
+    L.faces = lat->faces.as<uint32_t>();
+    L.hist = lat->hist.as<double>();
+    L.counter = lat->counter.as<uint32_t>();
+    L.tau0 = 0.5 + 3.0 * su->nu;   // lattice.rs:328
+    L.smag2 = su->smagorinsky * su->smagorinsky;
+    L.sponge_tau = su->sponge_tau;
+    L.sigma = su->sponge_sigma;
+    for (int k = 0; k < 3; ++k) {
+        L.u_inf[k] = su->u_inf[k];
+        L.u_init[k] = su->u_init[k];
+        L.grav[k] = su->body_force[k];
+        L.center[k] = su->center[k];
+        L.sp_lo[k] = (double)su->sponge[2 * k];
+        L.sp_hi[k] = (double)su->sponge[2 * k + 1];
+    }
+    L.abs_u = su->absorb_velocity != 0;
+    L.total_p = su->total_pressure_inlet != 0;
+    L.wall_model = su->wall_model != 0;
+    L.moving = 0;
+    L.forcing = (su->body_force[0] != 0.0 || su->body_force[1] != 0.0 || su->body_force[2] != 0.0) ? 1 : 0;
+    L.sponge = 0;
+    for (int k = 0; k < 6; ++k)
+        if (su->sponge[k]) L.sponge = 1;
+    L.nvert = lat->nvert;
+    L.verts = lat->verts.as<float4>();
+    L.vertex_offset = pr ? (float)pr->vertex_offset : 0.0f;
+    L.vacc = lat->vacc.as<float>();
+    L.npix = lat->npix;
+    L.slicedef = lat->slicedef.as<int4>();
+    L.sacc = lat->sacc.as<float>();
+    choose_shape(L, default_block_threads(), 0);
+    lat->partials.alloc((size_t)L.wall_blocks * 8 * 8, mem);
+    L.partials = lat->partials.as<double>();
+    lat->memory_bytes = mem;
+
+    hip_check(launch_init(L, st), "init kernel");
+    hip_check(hipStreamSynchronize(st), "setup");
+    *out = lat.release();
+    return SIMLAB_OK;
+}
+
+int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, double *ft, bool still) {
+    if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+    if (!still && nsteps && !steps) return fail(SIMLAB_ERR_INVALID, "null steps");
+    hip_check(hipSetDevice(lat->device), "hipSetDevice");
+    simlab_step_in idn{};
+    for (int k = 0; k < 3; ++k) {
+        idn.rot[k][k] = 1.0;
+        idn.rot_prev[k][k] = 1.0;
+    }
+    hipStream_t st = lat->stream;
+    if (lat->timing && lat->ev.size() < 2 * KMAX + 2) {
+        while (lat->ev.size() < 2 * KMAX + 2) {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            lat->ev.push_back(e);
+        }
+    }
+    lat->last_step_ms = 0.0;
+    lat->last_main_ms = 0.0;
+    lat->last_main_launches = 0;
+    size_t done = 0;
+    while (done < nsteps) {
+        const size_t chunk = std::min<size_t>(KMAX, nsteps - done);
+        if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX], st), "event");
+        for (size_t k = 0; k < chunk; ++k) {
+            const simlab_step_in &s = still ? idn : steps[done + k];
+            const uint64_t step = lat->steps_done + k;
+            const int par = (int)(step % 2);
+            StepArgs a;
+            pack_step(s, step, (uint32_t)k, a);
+            hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
+            if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
+            hip_check(launch_stream_collide(lat->L, par, a, st), "stream-collide kernel");
+            if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+            if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
+        }
+        if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
+        hip_check(hipMemcpyAsync(lat->hist_host, lat->hist.p, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st), "force copy");
+        hip_check(hipStreamSynchronize(st), "GPU wait failed");
+        if (lat->timing) {
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, lat->ev[2 * KMAX], lat->ev[2 * KMAX + 1]), "event time");
+            lat->last_step_ms += ms;
+            for (size_t k = 0; k < chunk; ++k) {
+                hip_check(hipEventElapsedTime(&ms, lat->ev[2 * k], lat->ev[2 * k + 1]), "event time");
+                lat->last_main_ms += ms;
+            }
+            lat->last_main_launches += chunk;
+        }
+        if (ft)
+            for (size_t k = 0; k < chunk; ++k)
+                for (int j = 0; j < 6; ++j) ft[(done + k) * 6 + j] = lat->hist_host[k * HIST_STRIDE + j];
+        lat->steps_done += chunk;
+        done += chunk;
+    }
+    return SIMLAB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *simlab_cfd_last_error(void) { return simlab::g_err.c_str(); }
+
+int simlab_cfd_gpu_info(char *buf, size_t len) {
+    return guarded([&]() -> int {
+        if (!buf || len == 0) return fail(SIMLAB_ERR_INVALID, "null buffer");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            hipGetLastError();
+            return fail(SIMLAB_ERR_GPU, "no usable GPU: no HIP device found");
+        }
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t p;
+        hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+        std::snprintf(buf, len, "%s \xc2\xb7 HIP \xc2\xb7 %s \xc2\xb7 %.1f GB max buffer", p.name, p.gcnArchName,
+                      (double)p.totalGlobalMem / 1e9);
+        return SIMLAB_OK;
+    });
+}
+
+uint64_t simlab_cfd_max_cells(int32_t storage) {
+    // layout limit: 32-bit cell indices with signed neighbour offsets
+    (void)storage;
+    return (1ull << 31) - 1;
+}
+
+int simlab_cfd_lattice_new(const simlab_lattice_setup *setup, const simlab_geometry *geom, const simlab_probes *probes,
+                           simlab_lattice **out) {
+    return guarded([&]() -> int {
+        try {
+            return new_lattice(setup, geom, probes, out);
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+void simlab_cfd_lattice_free(simlab_lattice *lat) {
+    if (!lat) return;
+    hipSetDevice(lat->device);
+    delete lat;
+}
+
+int simlab_cfd_lattice_info(const simlab_lattice *lat, simlab_lattice_info *out) {
+    if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof *out);
+    out->steps_done = lat->steps_done;
+    out->cells = lat->ncell;
+    out->memory_bytes = lat->memory_bytes;
+    for (int k = 0; k < 3; ++k) out->dims[k] = lat->setup.dims[k];
+    out->nrec = lat->nrec;
+    out->nface = lat->nface;
+    out->npix = lat->npix;
+    out->nvert = lat->nvert;
+    out->storage = lat->storage;
+    out->moving = lat->moving ? 1 : 0;
+    out->tau0 = lat->L.tau0;
+    return SIMLAB_OK;
+}
+
+int simlab_cfd_lattice_run(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, double *ft_out) {
+    return guarded([&]() -> int {
+        try {
+            return run_steps(lat, steps, nsteps, ft_out, false);
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_run_still(simlab_lattice *lat, size_t nsteps, double *ft_out) {
+    return guarded([&]() -> int {
+        try {
+            return run_steps(lat, nullptr, nsteps, ft_out, true);
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_macros(simlab_lattice *lat, float *out) {
+    return guarded([&]() -> int {
+        if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+        if (!lat->setup.debug_dump) return fail(SIMLAB_ERR_STATE, "macros() needs debug_dump");   // lattice.rs:820-822
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            const int par = (int)(lat->steps_done % 2);   // lattice.rs:824
+            hip_check(launch_macros(lat->L, par, lat->dump.as<float4>(), lat->stream), "dump kernel");
+            hip_check(hipMemcpyAsync(out, lat->dump.p, lat->ncell * 16, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_records(simlab_lattice *lat, uint32_t *out) {
+    return guarded([&]() -> int {
+        if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            const size_t nr = lat->nrec, rb = real_bytes(lat->storage);
+            std::vector<uint32_t> mask(nr), cell(nr), q(9 * nr);
+            std::vector<unsigned char> fpost(18 * nr * rb), mac(4 * nr * rb), ftv(6 * nr * rb);
+            hipStream_t st = lat->stream;
+            auto down = [&](void *dst, const DevBuf &b, size_t bytes) {
+                if (bytes) hip_check(hipMemcpyAsync(dst, b.p, bytes, hipMemcpyDeviceToHost, st), "GPU read failed");
+            };
+            down(mask.data(), lat->rmask, nr * 4);
+            down(cell.data(), lat->rcell, nr * 4);
+            down(q.data(), lat->rq, 9 * nr * 4);
+            down(fpost.data(), lat->rfpost, fpost.size());
+            down(mac.data(), lat->rmac, mac.size());
+            down(ftv.data(), lat->rft, ftv.size());
+            hip_check(hipStreamSynchronize(st), "GPU wait failed");
+            auto getf = [&](const std::vector<unsigned char> &v, size_t idx) -> uint32_t {
+                float f;
+                if (rb == 8) {
+                    double d;
+                    std::memcpy(&d, &v[idx * 8], 8);
+                    f = (float)d;
+                } else {
+                    std::memcpy(&f, &v[idx * 4], 4);
+                }
+                uint32_t u;
+                std::memcpy(&u, &f, 4);
+                return u;
+            };
+            const size_t nout = std::max<size_t>(nr, 1);
+            std::memset(out, 0, nout * 40 * 4);
+            for (size_t r = 0; r < nr; ++r) {   // shader.rs:62-69 layout
+                uint32_t *w = out + r * 40;
+                w[0] = mask[r];
+                w[1] = cell[r];
+                for (int k = 0; k < 9; ++k) w[2 + k] = q[(size_t)k * nr + r];
+                for (int i = 0; i < 18; ++i) w[11 + i] = getf(fpost, (size_t)i * nr + r);
+                for (int k = 0; k < 4; ++k) w[29 + k] = getf(mac, (size_t)k * nr + r);
+                for (int k = 0; k < 6; ++k) w[33 + k] = getf(ftv, (size_t)k * nr + r);
+            }
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_reset_samples(simlab_lattice *lat) {
+    return guarded([&]() -> int {
+        if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            hip_check(hipMemsetAsync(lat->vacc.p, 0, lat->vacc.bytes, lat->stream), "memset");
+            hip_check(hipMemsetAsync(lat->sacc.p, 0, lat->sacc.bytes, lat->stream), "memset");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_vertex_means(simlab_lattice *lat, float *out) {
+    return guarded([&]() -> int {
+        if (!lat || (!out && lat->nvert)) return fail(SIMLAB_ERR_INVALID, "null argument");
+        if (!lat->nvert) return SIMLAB_OK;
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            std::vector<float> v((size_t)lat->nvert * 2);
+            hip_check(hipMemcpyAsync(v.data(), lat->vacc.p, v.size() * 4, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            for (uint32_t i = 0; i < lat->nvert; ++i) out[i] = v[2 * i + 1] > 0.0f ? v[2 * i] / v[2 * i + 1] : std::nanf("");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_slice_means(simlab_lattice *lat, float *out) {
+    return guarded([&]() -> int {
+        if (!lat || (!out && lat->npix)) return fail(SIMLAB_ERR_INVALID, "null argument");
+        if (!lat->npix) return SIMLAB_OK;
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            std::vector<float> v((size_t)lat->npix * 5);
+            hip_check(hipMemcpyAsync(v.data(), lat->sacc.p, v.size() * 4, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            for (uint32_t i = 0; i < lat->npix; ++i) {
+                const float c = v[5 * i + 4];
+                for (int k = 0; k < 4; ++k) out[4 * i + k] = c > 0.0f ? v[5 * i + k] / c : std::nanf("");
+            }
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_mass(simlab_lattice *lat, double *mass) {
+    return guarded([&]() -> int {
+        if (!lat || !mass) return fail(SIMLAB_ERR_INVALID, "null argument");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            const int par = (int)(lat->steps_done % 2);
+            hip_check(launch_mass(lat->L, par, lat->mass_part.as<double>(), 1024, lat->mass_out.as<double>(), lat->stream), "mass kernel");
+            hip_check(hipMemcpyAsync(mass, lat->mass_out.p, 8, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+        const size_t want = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+        if (bytes != want) return fail(SIMLAB_ERR_INVALID, "population buffer has the wrong size");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            hip_check(hipMemcpyAsync(out, lat->pops.p, want, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!lat || !in) return fail(SIMLAB_ERR_INVALID, "null argument");
+        const size_t want = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+        if (bytes != want) return fail(SIMLAB_ERR_INVALID, "population buffer has the wrong size");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            hip_check(hipMemcpyAsync(lat->pops.p, in, want, hipMemcpyHostToDevice, lat->stream), "GPU write failed");
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable) {
+    if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+    lat->timing = enable != 0;
+    return SIMLAB_OK;
+}
+
+int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms, double *main_ms, uint64_t *main_launches) {
+    if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+    if (step_ms) *step_ms = lat->last_step_ms;
+    if (main_ms) *main_ms = lat->last_main_ms;
+    if (main_launches) *main_launches = lat->last_main_launches;
+    return SIMLAB_OK;
+}
+
+}  // extern "C"

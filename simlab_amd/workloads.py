@@ -1,0 +1,102 @@
+"""Named workloads: the reference's own test lattices (crates/cfd/tests/
+channel.rs:22-73, momentum.rs:9-37, examples/bench.rs) and the sphere-in-a-
+channel cases the throughput and accuracy figures are quoted on. Each
+builder returns (LatticeSetup, Geometry, meta) ready for `Lattice`."""
+from __future__ import annotations
+
+import math
+
+from . import FaceBc, Geometry, LatticeSetup, Storage, voxelize_static
+from .shapes import cuboid, merge, sphere
+
+
+def static_geometry(mesh, dims) -> Geometry:
+    return Geometry.static(voxelize_static(mesh.positions, mesh.triangles, dims))
+
+
+def poiseuille(h: float, storage: Storage = Storage.FP32, umax: float = 0.02):
+    """Plane Poiseuille flow between plates off the grid (channel.rs:39-73):
+    3 x ny x 3, all faces periodic, plates at y = 1.37 and 1.37 + h."""
+    ya = 1.37
+    yb = ya + h
+    ny = int(math.ceil(yb)) + 2
+    nx = nz = 3
+    lo = cuboid([-5.0, -10.0, -5.0], [nx + 5.0, ya, nz + 5.0])
+    hi = cuboid([-5.0, yb, -5.0], [nx + 5.0, ny + 10.0, nz + 5.0])
+    nu = 0.1
+    g = umax * 8.0 * nu / (h * h)
+    setup = LatticeSetup(dims=(nx, ny, nz), faces=(FaceBc.PERIODIC,) * 6, nu=nu, body_force=(g, 0.0, 0.0),
+                         storage=storage, debug_dump=True)
+    geom = static_geometry(merge(lo, hi), (nx, ny, nz))
+    steps = int(3.0 * h * h / nu)
+    return setup, geom, dict(ya=ya, yb=yb, ny=ny, nx=nx, nz=nz, nu=nu, g=g, steps=steps)
+
+
+def poiseuille_error(meta, mac, hist_last):
+    """L2 relative error of u_x on the x=1, z=1 column and the wall-force balance (channel.rs:56-72)."""
+    import numpy as np
+    ya, yb, ny, nx, nu, g = meta["ya"], meta["yb"], meta["ny"], meta["nx"], meta["nu"], meta["g"]
+    e2 = r2 = 0.0
+    for y in range(ny):
+        yf = float(y)
+        if yf <= ya or yf >= yb:
+            continue
+        exact = g / (2.0 * nu) * (yf - ya) * (yb - yf)
+        c = (1 * ny + y) * nx + 1
+        u = float(mac[c][1])
+        e2 += (u - exact) ** 2
+        r2 += exact * exact
+    rho = mac[:, 0]
+    fluid = float(np.sum(rho[~np.isnan(rho)].astype(np.float64)))
+    bal = (hist_last[0] - g * fluid) / (g * fluid)
+    return math.sqrt(e2 / r2), bal
+
+
+def ball_box(n: int, r: float, nu: float, u0: float, g: float, faces, storage: Storage = Storage.FP32):
+    """momentum.rs:9-37: an n^3 box with a sphere at (c+0.3, c-0.6, c+0.17)."""
+    c = n / 2.0
+    m = sphere([c + 0.3, c - 0.6, c + 0.17], r, 5)
+    setup = LatticeSetup(dims=(n, n, n), faces=tuple(faces), nu=nu, u_inf=(u0, 0.0, 0.0), u_init=(u0, 0.0, 0.0),
+                         body_force=(g, 0.0, 0.0), storage=storage, center=(c, c, c), debug_dump=True)
+    return setup, static_geometry(m, (n, n, n))
+
+
+def channel_walls(dims):
+    """Solid slabs whose surfaces sit half a cell inside the y and z edges
+    (y, z = 0.5 and n - 1.5): halfway bounce-back channel walls."""
+    nx, ny, nz = dims
+    return [
+        cuboid([-5.0, -10.0, -5.0], [nx + 5.0, 0.5, nz + 5.0]),
+        cuboid([-5.0, ny - 1.5, -5.0], [nx + 5.0, ny + 10.0, nz + 5.0]),
+        cuboid([-5.0, -5.0, -10.0], [nx + 5.0, ny + 5.0, 0.5]),
+        cuboid([-5.0, -5.0, nz - 1.5], [nx + 5.0, ny + 5.0, nz + 10.0]),
+    ]
+
+
+def sphere_channel(dims=(128, 64, 64), radius=8.0, x0=32.0, re=100.0, u=0.05, cs=0.0,
+                   storage: Storage = Storage.FP32, debug_dump=False, subdiv=5):
+    """Flow past a sphere in a walled channel: -x velocity inlet (u), +x open
+    outlet, y/z walls as halfway bounce-back slabs (periodic behind them),
+    viscosity from Re on the sphere diameter. 128x64x64 r=8 Re=100 is the
+    reference-sized case; 256x128x128 r=16 Re=1000 Cs=0.1 the large one."""
+    nx, ny, nz = dims
+    cy, cz = (ny - 1) / 2.0, (nz - 1) / 2.0
+    nu = u * 2.0 * radius / re
+    mesh = merge(sphere([x0, cy, cz], radius, subdiv), *channel_walls(dims))
+    setup = LatticeSetup(dims=tuple(dims),
+                         faces=(FaceBc.VELOCITY, FaceBc.OPEN, FaceBc.PERIODIC, FaceBc.PERIODIC, FaceBc.PERIODIC, FaceBc.PERIODIC),
+                         nu=nu, smagorinsky=cs, u_inf=(u, 0.0, 0.0), u_init=(u, 0.0, 0.0), storage=storage,
+                         center=(x0, cy, cz), debug_dump=debug_dump, total_pressure_inlet=False)
+    geom = static_geometry(mesh, dims)
+    cd_scale = 1.0 / (0.5 * u * u * math.pi * radius * radius)
+    return setup, geom, dict(nu=nu, tau0=0.5 + 3.0 * nu, cd_scale=cd_scale, radius=radius, u=u, re=re)
+
+
+def sphere_channel_small(storage: Storage = Storage.FP32, debug_dump=False):
+    """128x64x64, sphere r=8 at x=32, u=0.05, Re=100, Cs=0 (2000 steps in full)."""
+    return sphere_channel((128, 64, 64), 8.0, 32.0, 100.0, 0.05, 0.0, storage, debug_dump)
+
+
+def sphere_channel_large(storage: Storage = Storage.FP32, debug_dump=False):
+    """256x128x128, sphere r=16 at x=64, u=0.05, Re=1000, Smagorinsky Cs=0.1 (5000 steps in full)."""
+    return sphere_channel((256, 128, 128), 16.0, 64.0, 1000.0, 0.05, 0.1, storage, debug_dump)

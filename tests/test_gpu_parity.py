@@ -1,0 +1,185 @@
+"""GPU parity: the HIP lattice (through the C ABI) against the CPU oracle on
+identical inputs, per storage. Tolerances are stated per test:
+
+* FP64 storage vs the FP64 oracle: populations within 1e-12 (absolute, on
+  f~ = f - w ~ 1e-2), forces within 1e-10 relative.
+* FP32 vs the FP32 oracle: after 1 step populations within 2e-7 absolute
+  (a few f32 ulps: fused multiply-adds and a reordered moment sum); after
+  200 steps velocity max |du| <= 1e-4 * u_ref, forces within 1e-3 relative.
+* FP16 / BF16 vs the oracle with the same storage and the same dither
+  stream: after 1 step >= 99 % of the stored 16-bit values identical and the
+  rest within one unit in the last place.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import simlab_amd as sa
+from simlab_amd import workloads as W
+from simlab_amd.shapes import merge, sphere
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+STORE = {sa.Storage.FP32: O.F32, sa.Storage.FP16: O.F16, sa.Storage.BF16: O.BF16, sa.Storage.FP64: O.F64}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def both(setup, geom, probes=None):
+    lat = sa.Lattice(setup, geom, probes)
+    ora = O.OracleLattice(setup, geom, probes, store=STORE[setup.storage])
+    return lat, ora
+
+
+def small_channel(storage, dims=(48, 24, 20), u=0.05, re=40.0, cs=0.0, radius=4.0, x0=14.0):
+    return W.sphere_channel(dims, radius, x0, re, u, cs, storage, debug_dump=True, subdiv=3)
+
+
+def test_gpu_info_names_the_device():
+    info = sa.gpu_info()
+    say("gpu_info:", info)
+    assert "gfx9" in info
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP64, sa.Storage.FP32])
+def test_one_step_populations_match_oracle(storage):
+    setup, geom, _ = small_channel(storage)
+    say("case one_step", storage.name, setup.dims)
+    lat, ora = both(setup, geom)
+    assert lat.nrec == ora.nrec and lat.nface == ora.nface
+    p0g, p0o = lat.get_pops(), ora.get_pops()
+    assert np.array_equal(p0g, p0o) if storage == sa.Storage.FP64 else np.max(np.abs(p0g - p0o)) < 1e-9
+    for k in range(2):   # both parities
+        hg = lat.run_still(1)
+        ho = ora.run_still(1)
+        pg, po = lat.get_pops().astype(np.float64), ora.get_pops().astype(np.float64)
+        tol = 1e-12 if storage == sa.Storage.FP64 else 2e-7
+        d = np.max(np.abs(pg - po))
+        say(f"  step {k + 1}: max |dpop| = {d:.3e}, F gpu {hg[0][:3]} oracle {ho[0][:3]}")
+        assert d < tol
+        ftol = 1e-10 if storage == sa.Storage.FP64 else 1e-4
+        assert np.allclose(hg, ho, rtol=ftol, atol=ftol * np.max(np.abs(ho)))
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP64, sa.Storage.FP32])
+def test_channel_trajectory_matches_oracle(storage):
+    setup, geom, meta = small_channel(storage)
+    say("case trajectory", storage.name, setup.dims)
+    lat, ora = both(setup, geom)
+    n = 200
+    hg = lat.run_still(n)
+    ho = ora.run_still(n)
+    mg, mo = lat.macros().astype(np.float64), ora.macros().astype(np.float64)
+    assert np.array_equal(np.isnan(mg[:, 0]), np.isnan(mo[:, 0]))
+    ok = ~np.isnan(mo[:, 0])
+    du = np.abs(mg[ok, 1:] - mo[ok, 1:])
+    l2 = np.linalg.norm(mg[ok, 1:] - mo[ok, 1:]) / np.linalg.norm(mo[ok, 1:])
+    fd = np.max(np.abs(hg - ho)) / np.max(np.abs(ho))
+    say(f"  max|du|/u = {du.max() / meta['u']:.3e}  L2 rel = {l2:.3e}  force rel = {fd:.3e}  Cd gpu {hg[-1][0] * meta['cd_scale']:.5f} oracle {ho[-1][0] * meta['cd_scale']:.5f}")
+    if storage == sa.Storage.FP64:
+        assert du.max() < 1e-11 and fd < 1e-9
+    else:
+        assert du.max() < 1e-4 * meta["u"] and l2 < 1e-4 and fd < 1e-3
+    rg, ro = lat.records(), ora.records()
+    assert np.array_equal(rg[:, :11], ro[:, :11])   # mask, cell, q words bit-exact
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
+def test_half_storage_one_step_bits(storage):
+    setup, geom, _ = small_channel(storage)
+    say("case half_bits", storage.name, setup.dims)
+    lat, ora = both(setup, geom)
+    assert np.array_equal(lat.get_pops(), ora.get_pops())   # init is exact
+    for k in range(2):
+        lat.run_still(1)
+        ora.run_still(1)
+        pg, po = lat.get_pops().astype(np.int64), ora.get_pops().astype(np.int64)
+        diff = np.abs(pg - po)
+        frac = np.mean(diff == 0)
+        say(f"  step {k + 1}: identical {frac:.5f}, max diff {diff.max()} ulp")
+        assert frac > 0.99 and diff.max() <= 1
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
+def test_half_storage_trajectory_close_to_fp64(storage):
+    setup, geom, meta = small_channel(storage)
+    say("case half_traj", storage.name, setup.dims)
+    lat = sa.Lattice(setup, geom)
+    s64, g64, _ = small_channel(sa.Storage.FP64)
+    ref = sa.Lattice(s64, g64)
+    n = 400
+    hg = lat.run_still(n)
+    hr = ref.run_still(n)
+    cd_g = hg[n // 2:, 0].mean() * meta["cd_scale"]
+    cd_r = hr[n // 2:, 0].mean() * meta["cd_scale"]
+    mg, mr = lat.macros().astype(np.float64), ref.macros().astype(np.float64)
+    ok = ~np.isnan(mr[:, 0])
+    l2 = np.linalg.norm(mg[ok, 1:] - mr[ok, 1:]) / np.linalg.norm(mr[ok, 1:])
+    say(f"  Cd {storage.name} {cd_g:.5f} vs FP64 {cd_r:.5f} ({(cd_g / cd_r - 1) * 100:+.3f} %), velocity L2 rel {l2:.2e}")
+    bar = 0.005 if storage == sa.Storage.FP16 else 0.02
+    assert abs(cd_g / cd_r - 1) < bar
+    assert l2 < (2e-3 if storage == sa.Storage.FP16 else 1e-2)
+
+
+def test_features_match_oracle():
+    # body force, sponges with absorbing layers, total-pressure inlet, wall
+    # model, Smagorinsky: 30 steps FP64 against the FP64 oracle
+    nx, ny, nz = 40, 20, 18
+    m = sphere([15.2, 9.7, 8.6], 3.6, 3)
+    geom = W.static_geometry(m, (nx, ny, nz))
+    setup = sa.LatticeSetup(dims=(nx, ny, nz), faces=(0, 1, 1, 1, 2, 2), nu=0.02, smagorinsky=0.16,
+                            u_inf=(0.06, 0.01, 0.0), u_init=(0.05, 0.0, 0.0), body_force=(1e-6, 0.0, 2e-7),
+                            sponge=(3, 5, 2, 2, 0, 0), sponge_tau=0.9, sponge_sigma=0.1, absorb_velocity=True,
+                            total_pressure_inlet=True, storage=sa.Storage.FP64, center=(15.2, 9.7, 8.6),
+                            debug_dump=True, wall_model=True)
+    setup.faces = [sa.FaceBc(f) for f in setup.faces]
+    say("case features FP64")
+    lat, ora = both(setup, geom)
+    hg = lat.run_still(30)
+    ho = ora.run_still(30)
+    pg, po = lat.get_pops(), ora.get_pops()
+    d = np.max(np.abs(pg - po))
+    say(f"  max |dpop| {d:.3e}, force rel {np.max(np.abs(hg - ho)) / np.max(np.abs(ho)):.3e}")
+    assert d < 1e-11
+    assert np.allclose(hg, ho, rtol=1e-9, atol=1e-12 * np.max(np.abs(ho)))
+
+
+def test_probes_match_oracle():
+    setup, geom, meta = small_channel(sa.Storage.FP32)
+    verts = [((4.5 * np.cos(a), 4.5 * np.sin(a), 0.3), (np.cos(a), np.sin(a), 0.0)) for a in np.linspace(0, 6, 12)]
+    probes = sa.Probes(verts=verts, slices=[sa.SliceDef((2, 2, 10), (1, 0, 0), (0, 1, 0), 40, 18)], vertex_offset=1.0)
+    say("case probes")
+    lat, ora = both(setup, geom, probes)
+    steps = [sa.StepIn.still(k % 3 == 0) for k in range(30)]
+    lat.run(steps)
+    ora.run(steps)
+    vg, vo = lat.vertex_means(), ora.vertex_means()
+    sg, so = lat.slice_means(), ora.slice_means()
+    assert np.allclose(vg, vo, rtol=1e-5, equal_nan=True)
+    assert np.allclose(sg, so, rtol=1e-4, atol=1e-6, equal_nan=True)
+    lat.reset_samples()
+    assert np.all(np.isnan(lat.vertex_means()))
+
+
+def test_mass_is_conserved_in_a_periodic_box():
+    setup, geom = W.ball_box(24, 4.0, 0.05, 0.03, 0.0, [sa.FaceBc.PERIODIC] * 6, storage=sa.Storage.FP64)
+    say("case mass FP64")
+    lat, ora = both(setup, geom)
+    m0 = lat.mass()
+    assert m0 == pytest.approx(ora.mass(), rel=1e-14)
+    lat.run_still(100)
+    m1 = lat.mass()
+    say(f"  mass {m0:.15g} -> {m1:.15g}")
+    assert abs(m1 - m0) / m0 < 1e-12
+
+
+def test_macros_needs_debug_dump():
+    setup = sa.LatticeSetup(dims=(8, 8, 8))
+    lat = sa.Lattice(setup)
+    with pytest.raises(sa.SimlabError, match="macros\\(\\) needs debug_dump"):
+        lat.macros()

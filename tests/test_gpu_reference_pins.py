@@ -1,0 +1,93 @@
+"""The reference's own lattice tests (crates/cfd/tests/channel.rs,
+momentum.rs) run on the MI355X lattice, with the reference's thresholds."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import simlab_amd as sa
+from simlab_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def poiseuille(h, storage, umax=0.02):
+    setup, geom, meta = W.poiseuille(h, storage, umax)
+    lat = sa.Lattice(setup, geom)
+    hist = lat.run_still(meta["steps"])
+    return W.poiseuille_error(meta, lat.macros(), hist[-1])
+
+
+def test_poiseuille_second_order():
+    errs = []
+    for h in (8.0, 16.0, 32.0):
+        say("case poiseuille FP32 H", h)
+        e, bal = poiseuille(h, sa.Storage.FP32)
+        say(f"  L2 {e:.3e} balance {bal:+.2e}")
+        assert abs(bal) < 2e-3
+        errs.append(e)
+    o2 = math.log2(errs[1] / errs[2])
+    assert errs[2] < 2e-3
+    assert o2 > 1.6 or errs[2] < 1e-4
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
+def test_poiseuille_half_storage(storage):
+    bar = 3e-3 if storage == sa.Storage.FP16 else 1e-2
+    for umax in (0.02, 0.06):
+        say("case poiseuille", storage.name, umax)
+        e, bal = poiseuille(16.0, storage, umax)
+        say(f"  L2 {e:.3e} balance {bal:+.2e}")
+        assert e < bar
+
+
+def test_momentum_lost_by_fluid_equals_force_on_body():
+    n = 40
+    setup, geom = W.ball_box(n, 6.0, 0.05, 0.05, 0.0, [sa.FaceBc.PERIODIC] * 6)
+    say("case momentum balance")
+    lat = sa.Lattice(setup, geom)
+
+    def pmom():
+        m = lat.macros()
+        ok = ~np.isnan(m[:, 0])
+        return float(np.sum((m[ok, 0] * m[ok, 1]).astype(np.float64)))
+
+    p0 = pmom()
+    h = lat.run_still(600)
+    p1 = pmom()
+    impulse = float(h[:, 0].sum())
+    err = (p0 - p1 - impulse) / impulse
+    say(f"  mismatch {err:+.3e}")
+    assert impulse > 0 and abs(err) < 5e-3
+
+
+def test_periodic_sphere_array_stokes_drag():
+    n, r, nu, g = 48, 6.0, 1.0 / 6.0, 1e-7
+    setup, geom = W.ball_box(n, r, nu, 0.0, g, [sa.FaceBc.PERIODIC] * 6)
+    say("case stokes array")
+    lat = sa.Lattice(setup, geom)
+    steps = int(4.0 * n * n / nu)
+    h = lat.run_still(steps)
+    mac = lat.macros()
+    ok = ~np.isnan(mac[:, 0])
+    ux = float(np.sum((mac[ok, 0] * mac[ok, 1]).astype(np.float64))) / n ** 3
+    f = h[-1][0]
+    c = 4.0 / 3.0 * math.pi * r ** 3 / n ** 3
+    k_ref = 1.0 / (1.0 - 1.7601 * c ** (1 / 3) + c - 1.5593 * c * c + 3.9799 * c ** (8 / 3) - 3.0734 * c ** (10 / 3))
+    k = f / (6.0 * math.pi * nu * r * ux)
+    say(f"  K {k:.4f} vs Hasimoto/Sangani-Acrivos {k_ref:.4f} ({(k / k_ref - 1) * 100:+.2f} %)")
+    assert abs(k / k_ref - 1.0) < 0.04
+
+
+def test_uniform_states_feel_no_force():
+    setup, geom = W.ball_box(32, 5.0, 0.05, 0.0, 0.0, [sa.FaceBc.OPEN] * 6)
+    say("case zero force")
+    lat = sa.Lattice(setup, geom)
+    h = lat.run_still(50)
+    assert np.all(np.abs(h) < 1e-4)
