@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Throughput and accuracy of the MI355X D3Q19 lattice on the 256x128x128
+sphere-in-a-channel workload (sphere r=16 at x=64, Re 1000 on its diameter,
+Smagorinsky Cs=0.1, -x velocity inlet u=0.05, +x open outlet, halfway
+bounce-back channel walls), the configuration the headline metric is quoted
+on.
+
+One step = one full lattice update (face kernel + stream-collide with the
+wall records and the FP64 force reduction). MLUPS counts all nx*ny*nz cells
+per step, as the reference does (crates/cfd/examples/bench.rs:25).
+
+Reported (one JSON line):
+  value            FP16-storage MLUPS (wall clock over the timed steps)
+  roofline         the bulk stream-collide kernel (k_bulk, the cells with no
+                   wall record and no face duty): algorithmic bytes per launch
+                   (2*19*s + 4 bytes per cell it updates, s = 2 for FP16) / its
+                   average duration from HIP events on the stream it runs on;
+                   storages.*.roofline_frac is the whole-step figure
+                   (MLUPS over all cells x bytes per cell / 8 TB/s)
+  fp32 / bf16      the same for FP32 and BF16 storage
+  drag             time-averaged drag coefficient of a 5000-step run per
+                   storage and its error against the FP64 run
+  cpu_baseline     the CPU oracle (C++ restatement of the reference kernels)
+                   on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS and % HBM roofline (FP16-store vs FP32) at stated drag error vs FP64"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "sphere-channel 256x128x128, sphere r=16 at x=64, Re=1000, Cs=0.1, u=0.05, halfway-bounce-back walls"
+
+
+def bytes_per_cell(storage):
+    return 2 * 19 * storage.pop_bytes() + 4
+
+
+def timed_run(sa, W, storage, steps, warmup):
+    setup, geom, meta = W.sphere_channel_large(storage)
+    lat = sa.Lattice(setup, geom)
+    lat.run_still(warmup)
+    t0 = time.perf_counter()
+    lat.run_still(steps)                      # returns after the stream is synchronized
+    wall = time.perf_counter() - t0
+    lat.set_timing(True)
+    lat.run_still(steps)                      # second pass: per-launch HIP events
+    step_ms, main_ms, launches = lat.timing()
+    n = setup.cells()
+    res = {
+        "mlups": n * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "device_ms_per_step": step_ms / steps,
+        "kernel_us": main_ms / launches * 1e3,
+        "nrec": lat.nrec,
+    }
+    res["bulk_cells"] = lat.bulk_cells
+    b = bytes_per_cell(storage) * lat.bulk_cells   # algorithmic bytes of one bulk launch
+    res["kernel_gbs"] = b / (main_ms / launches * 1e-3) / 1e9
+    res["kernel_roofline_frac"] = res["kernel_gbs"] / HBM_PEAK_GBS
+    res["roofline_frac"] = res["mlups"] * 1e6 * bytes_per_cell(storage) / (HBM_PEAK_GBS * 1e9)
+    del lat
+    return res, n
+
+
+def drag_run(sa, W, storage, steps):
+    setup, geom, meta = W.sphere_channel_large(storage)
+    lat = sa.Lattice(setup, geom)
+    h = lat.run_still(steps)
+    m1 = lat.mass()
+    window = h[steps // 2:]
+    cd = float(window[:, 0].mean() * meta["cd_scale"])
+    cl = float(window[:, 2].mean() * meta["cd_scale"])
+    del lat
+    return {"cd": cd, "cl": cl, "cd_last": float(h[-1, 0] * meta["cd_scale"]), "mass_end": m1}
+
+
+def cpu_baseline(sa, W, target_s=12.0):
+    from oracle import pyoracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    setup, geom, meta = W.sphere_channel_large(sa.Storage.FP32)
+    lat = O.OracleLattice(setup, geom, store=O.F32, threads=threads)
+    lat.run_still(1)
+    t0 = time.perf_counter()
+    lat.run_still(2)
+    per = (time.perf_counter() - t0) / 2
+    k = max(2, min(200, int(target_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    lat.run_still(k)
+    dt = time.perf_counter() - t0
+    n = setup.cells()
+    return {"value": n * k / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C++ oracle (restatement of the reference kernels), FP32 storage, {WORKLOAD}: {k} steps after 3 warm-up steps, {dt:.1f} s"}
+
+
+def traffic_from_profile():
+    """HBM bytes per stream-collide launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (None if absent)."""
+    p = os.path.join(ROOT, "profiles", "stream_collide_sphere256_fp16_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--drag-steps", type=int, default=5000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-drag", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the lattice is single-device (the multi-GPU slab split is a design "
+              f"sketch, not built); --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import simlab_amd as sa
+    from simlab_amd import workloads as W
+
+    info = sa.gpu_info()
+    print(f"# {info}", file=sys.stderr, flush=True)
+    storages = [s for s in (args.only.split(",") if args.only else ["fp16", "fp32", "bf16"]) if s]
+    results = {}
+    n = None
+    for name in storages:
+        st = {"fp16": sa.Storage.FP16, "fp32": sa.Storage.FP32, "bf16": sa.Storage.BF16}[name]
+        print(f"# timing {name}: {args.warmup} warm-up + {args.steps} steps", file=sys.stderr, flush=True)
+        results[name], n = timed_run(sa, W, st, args.steps, args.warmup)
+        print(f"#   {name}: {results[name]['mlups']:.0f} MLUPS, kernel {results[name]['kernel_us']:.1f} us, "
+              f"{results[name]['kernel_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    drag = None
+    if not args.no_drag:
+        drag = {}
+        for name, st in (("fp64", sa.Storage.FP64), ("fp32", sa.Storage.FP32), ("fp16", sa.Storage.FP16), ("bf16", sa.Storage.BF16)):
+            print(f"# drag run {name}: {args.drag_steps} steps", file=sys.stderr, flush=True)
+            drag[name] = drag_run(sa, W, st, args.drag_steps)
+        for name in ("fp32", "fp16", "bf16"):
+            drag[name]["cd_err_vs_fp64"] = drag[name]["cd"] / drag["fp64"]["cd"] - 1.0
+        drag["window"] = f"mean over steps {args.drag_steps // 2}..{args.drag_steps}"
+
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sa, W)
+
+    head = results.get("fp16") or next(iter(results.values()))
+    head_name = "fp16" if "fp16" in results else next(iter(results))
+    head_st = {"fp16": sa.Storage.FP16, "fp32": sa.Storage.FP32, "bf16": sa.Storage.BF16}[head_name]
+    out = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": head_name,
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "dims": [256, 128, 128], "cells": n, "storage": head_name,
+                   "bytes_per_cell": bytes_per_cell(head_st), "parallelism": "single-gpu"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(head["kernel_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(),
+        },
+        "cpu_baseline": cpu,
+        "storages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in results.items()},
+    }
+    if "fp16" in results and "fp32" in results:
+        out["fp16_vs_fp32_speedup"] = round(results["fp16"]["mlups"] / results["fp32"]["mlups"], 3)
+    if drag is not None:
+        out["drag"] = drag
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

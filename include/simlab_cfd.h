@@ -153,6 +153,8 @@ typedef struct simlab_lattice_info {
     int32_t storage;             /* effective storage                       */
     int32_t moving;
     double tau0;
+    uint64_t bulk_cells;         /* cells of the plain (no wall, no face
+                                    duty) stream-collide kernel            */
 } simlab_lattice_info;
 
 typedef struct simlab_lattice simlab_lattice;            /* opaque */
@@ -221,8 +223,9 @@ int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes);
 int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t bytes);
 
 /* Device timing of the last `run` call: total milliseconds of its steps and
- * the summed duration and count of the stream-collide launches, measured with
- * HIP events on the lattice's stream (timing must have been enabled). */
+ * the summed duration and count of the bulk stream-collide launches,
+ * measured with HIP events on the stream they run on (timing must have been
+ * enabled). */
 int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
 int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
                               double *main_ms, uint64_t *main_launches);

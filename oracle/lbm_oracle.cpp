@@ -119,8 +119,10 @@ float sround_ref(float x, uint32_t key, uint32_t seed) {
 
 // This library's dither stream for FP16/BF16 stores: one hash per cell (the
 // reference's mixing function applied to the cell index), then xorshift32
-// words; direction i takes 16 bits of word i/2. Stochastic rounding itself is
-// "add the dither below the kept mantissa bits, truncate".
+// words; direction i takes 16 bits of word i/2. Face-cell (guarded) stores
+// take the top 16 bits of the reference's per-store hash instead (key =
+// cell * 19 + slot direction). Stochastic rounding itself is "add the dither
+// below the kept mantissa bits, truncate".
 uint32_t cell_hash(uint32_t n, uint32_t seed) {
     uint32_t h = (n * 0x9E3779B1u) ^ (seed * 0x85EBCA77u + 0x632BE5ABu);
     h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
@@ -209,6 +211,10 @@ template <class T> struct Lat {
     uint32_t npix = 0;
     uint64_t steps_done = 0;
     int nthreads = 0;
+    // test mode: the face update of step k+1 done right after step k (the
+    // GPU library's order) instead of at the start of step k+1
+    bool fused = false;
+    bool first_of_run = true;
 
     // ---- slots (shader.rs:147-162) ----
     inline uint64_t slot(int i, uint64_t cell) const { return (uint64_t)i * N + cell; }
@@ -266,7 +272,14 @@ template <class T> struct Lat {
     }
     void scatter_post(uint64_t n, int q, const T f[19], bool guard, uint32_t seed) {
         uint32_t x, y, z; coords(n, x, y, z);
-        uint16_t dz[19]; if (store == ST_F16 || store == ST_BF16) dither16((uint32_t)n, seed, dz);
+        uint16_t dz[19];
+        if (store == ST_F16 || store == ST_BF16) {
+            if (guard) {   // face cells / refills: one hash per store, key n*19 + slot direction
+                for (int j = 0; j < 19; ++j) dz[j] = (uint16_t)(cell_hash((uint32_t)n * 19u + (uint32_t)j, seed) >> 16);
+            } else {
+                dither16((uint32_t)n, seed, dz);
+            }
+        }
         for (int i = 0; i < 19; ++i) {
             if (q == 0 || i == 0) { st(opp(i), n, f[i], (uint32_t)n, seed, dz); continue; }
             if (guard) {   // shader.rs:287-297
@@ -814,13 +827,24 @@ template <class T> struct Lat {
             qclear();
         }
         int64_t nf = (int64_t)faces.size();
+        if (!fused || first_of_run) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
-        for (int64_t k = 0; k < nf; ++k) face_gather((uint32_t)k, p);
+            for (int64_t k = 0; k < nf; ++k) face_gather((uint32_t)k, p);
 #pragma omp parallel for schedule(static) num_threads(nthreads)
-        for (int64_t k = 0; k < nf; ++k) face_write((uint32_t)k, p, s.seed);
+            for (int64_t k = 0; k < nf; ++k) face_write((uint32_t)k, p, s.seed);
+            if (fused) {   // inputs of the step after this one too (after the gather above)
+                for (int64_t k = 0; k < nf; ++k) face_gather((uint32_t)k, 1 - p);
+                for (int64_t k = 0; k < nf; ++k) face_write((uint32_t)k, 1 - p, seed_of(steps_done + 1));
+            }
+        }
+        first_of_run = false;
         int64_t nn = (int64_t)N;
 #pragma omp parallel for schedule(static) num_threads(nthreads)
         for (int64_t n = 0; n < nn; ++n) main_cell((uint64_t)n, p, s, moving);
+        if (fused) {   // the next step's face update, right after this step
+            for (int64_t k = 0; k < nf; ++k) face_gather((uint32_t)k, 1 - p);
+            for (int64_t k = 0; k < nf; ++k) face_write((uint32_t)k, 1 - p, seed_of(steps_done + 1));
+        }
         double acc[6] = {0, 0, 0, 0, 0, 0};   // FP64 accumulation in record order
         for (const Rec<T> &R : recs)
             for (int j = 0; j < 6; ++j) acc[j] += (double)R.ft[j];
@@ -1015,9 +1039,15 @@ void oracle_free(void *hp) {
         else { auto *L = as<float>(h); expr; }              \
     } while (0)
 
+void oracle_set_fused(void *hp, int on) {
+    Handle *h = (Handle *)hp;
+    DISPATCH(h, L->fused = on != 0);
+}
+
 int oracle_run(void *hp, const simlab_step_in *steps, uint64_t nsteps, double *ft) {
     Handle *h = (Handle *)hp;
     double tmp[6];
+    DISPATCH(h, L->first_of_run = true);
     DISPATCH(h, for (uint64_t k = 0; k < nsteps; ++k) L->step(steps[k], ft ? ft + 6 * k : tmp));
     return 0;
 }
@@ -1028,6 +1058,7 @@ int oracle_run_still(void *hp, uint64_t nsteps, double *ft) {
     for (int k = 0; k < 3; ++k) { s.rot[k][k] = 1.0; s.rot_prev[k][k] = 1.0; }
     Handle *h = (Handle *)hp;
     double tmp[6];
+    DISPATCH(h, L->first_of_run = true);
     DISPATCH(h, for (uint64_t k = 0; k < nsteps; ++k) L->step(s, ft ? ft + 6 * k : tmp));
     return 0;
 }

@@ -67,6 +67,7 @@ def lib():
         L.oracle_enc_bf16_sr.argtypes = [C.c_float, C.c_uint16]
         L.oracle_aa_maps.argtypes = [P, C.c_int, P, P]
         L.oracle_collide_f64.argtypes = [P, C.c_double, C.c_double, P, P]
+        L.oracle_set_fused.argtypes = [V, C.c_int]
         _lib = L
     return _lib
 
@@ -156,6 +157,9 @@ class OracleLattice:
         out = np.zeros(max(self.nface, 1), np.uint32)
         self._lib.oracle_get_faces(self._h, _ptr(out))
         return out[: self.nface]
+
+    def set_fused(self, on: bool):
+        self._lib.oracle_set_fused(self._h, int(bool(on)))
 
     def mass(self):
         return float(self._lib.oracle_mass(self._h))

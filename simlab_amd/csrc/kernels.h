@@ -11,6 +11,7 @@ namespace simlab {
 
 constexpr uint32_t KMAX = 512;   // steps per device-to-host force copy (lattice.rs:134)
 constexpr uint32_t HIST_STRIDE = 8;   // doubles per step in the force history
+constexpr uint32_t SPECIAL_BLOCK = 256;
 
 // Device view of one lattice. Population slot of direction i and cell n is
 // element i * ncell + n of `pops` (SoA, x fastest, as shader.rs:147-152).
@@ -35,6 +36,7 @@ struct DevLattice {
     uint32_t nface;
     uint32_t *faces;
 
+
     double *hist;                // [KMAX][HIST_STRIDE]
     double *partials;            // [wall_blocks][8]
     uint32_t *counter;           // last-block detection, returns to 0 after each step
@@ -50,9 +52,7 @@ struct DevLattice {
     uint32_t by;                 // x-rows per block
     uint32_t xblocks;            // ceil(nx / bx)
     uint32_t yblocks;            // ceil(ny / by)
-    uint32_t wall_blocks;        // blocks that process the records (>= 1)
-    uint32_t wall_rows;          // grid rows taken by the wall blocks
-    uint32_t xcd_remap;          // remap block ids so neighbouring tiles share an XCD
+    uint32_t wall_blocks;        // 256-thread blocks that process the records (>= 1)
 
     // moving geometry (swept box + body-frame occupancy)
     int32_t box_lo[3];
@@ -86,8 +86,10 @@ struct StepArgs {
 };
 
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
+// standalone face update before a step of this parity (seed of that step)
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
-hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+// k_special on s_special, k_bulk on s_bulk (independent within a step)
+hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);

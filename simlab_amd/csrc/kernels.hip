@@ -7,15 +7,17 @@
 //  * AA pattern in place (shader.rs:5-13): a step of parity 0 reads (n, i)
 //    and writes (n, opp i); parity 1 reads (n - c_i, opp i) and writes
 //    (n + c_i, i). Every slot is touched by one thread per step;
-//  * one thread per cell, a block = whole x-rows (bx >= nx lanes, by rows),
-//    so the +-1 x-offset accesses of parity 1 stay inside one block (one CU
-//    and one XCD L2) and every wave reads/writes whole cache lines;
-//  * cells with a wall record are taken out of the bulk blocks and processed
-//    by "wall blocks" (scheduled first in the same launch) that walk the
-//    record list, so bulk waves never diverge on wall logic. The wall blocks
-//    also reduce the momentum-exchange force and torque in FP64 (wave
-//    shuffles -> LDS -> per-block partial -> last block sums the partials in
-//    block order, deterministic), replacing reduce1/reduce2 (shader.rs:689-734).
+//  * one thread per cell; the bulk kernel's block is `by` whole x-rows, so
+//    the +-1 x-offset accesses of parity 1 stay inside one block (one CU
+//    and one XCD L2) and waves read and write whole cache lines;
+//  * two launches per step: k_special (wall-record cells, cells that feed
+//    face cells, and the FP64 momentum-exchange reduction that replaces
+//    reduce1/reduce2, shader.rs:689-734) and k_bulk (every other fluid
+//    cell). Keeping the wall logic out of k_bulk keeps its register count
+//    (and so its occupancy) that of the plain collision;
+//  * the face update (shader.rs:630-687) is fused into the step before: the
+//    interior cell that a face cell copies its state from writes the face
+//    cell's populations right after its own collision;
 //  * FP16/BF16 stores round stochastically: one hash per cell, xorshift32
 //    words give 16 dither bits per stored direction, added below the kept
 //    mantissa bits before a round-toward-zero conversion.
@@ -35,6 +37,12 @@ constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 
 constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
 constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
 constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// flags word: bits 0-1 cell type (0 fluid, 1 solid, 2 velocity face, 3 open
+// face), bits 2-30 wall record index + 1
+constexpr uint32_t REC_MASK = 0x1FFFFFFFu;
+// wall record index + 1 of a flags word (0: none)
+__device__ __forceinline__ uint32_t rec_of(uint32_t fg) { return (fg >> 2) & REC_MASK; }
 
 template <class T> __device__ __forceinline__ constexpr T wgt(int i) {
     return i == 0 ? (T)(1.0 / 3.0) : (i <= 6 ? (T)(1.0 / 18.0) : (T)(1.0 / 36.0));
@@ -199,19 +207,24 @@ template <int i> __device__ __forceinline__ bool keeps_inside(const DevLattice &
     return ok;
 }
 
-template <class C, int Q, bool GUARD>
-__device__ __forceinline__ void scatter_post(const DevLattice &L, const Nbr &nb, const typename C::T f[19], uint32_t seed) {
+template <class C, int Q, bool GUARD, bool RT = false>
+__device__ __forceinline__ void scatter_post(const DevLattice &L, const Nbr &nb, typename C::T f[19], uint32_t seed) {
+    // RT: f[i] becomes the value as stored (decoded back), for face gathers
     Dither d;
     if constexpr (C::SR) d.init(nb.n, seed);
     sfor<19>([&](auto I) {
         constexpr int i = I;
         if constexpr (Q == 0 || i == 0) {
             constexpr int j = OPP[i];   // slot direction
-            popdir<C>(L, j)[nb.n] = enc<C>(f[i], C::SR ? d.template d16<j>() : 0u);
+            const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<j>() : 0u);
+            popdir<C>(L, j)[nb.n] = e;
+            if constexpr (RT) f[i] = dec<C>(e);
         } else {
             bool ok = true;
             if constexpr (GUARD) ok = keeps_inside<i>(L, nb);
-            if (ok) popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()] = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
+            const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
+            if (ok) popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()] = e;
+            if constexpr (RT) f[i] = dec<C>(e);
         }
     });
 }
@@ -314,12 +327,34 @@ template <class T> __device__ __forceinline__ T sponge_at(const DevLattice &L, u
     return fmin(fmax(s, (T)0), (T)1);
 }
 
+template <class T> struct FaceSrc {
+    T rho, ux, uy, uz, p[6];
+};
+
+// rho, u, Pi_neq of post-collision populations (shader.rs:645-653)
+template <class T> __device__ __forceinline__ FaceSrc<T> face_source(const DevLattice &L, const T f[19]) {
+    FaceSrc<T> s;
+    const Mom<T> mo = moments(f);
+    s.rho = (T)1 + mo.rho1;
+    s.ux = div_(mo.jx, s.rho) - (T)0.5 * (T)L.grav[0];
+    s.uy = div_(mo.jy, s.rho) - (T)0.5 * (T)L.grav[1];
+    s.uz = div_(mo.jz, s.rho) - (T)0.5 * (T)L.grav[2];
+    const T r3 = div_(mo.rho1, (T)3);
+    s.p[0] = mo.sxx - s.rho * s.ux * s.ux - r3;
+    s.p[1] = mo.syy - s.rho * s.uy * s.uy - r3;
+    s.p[2] = mo.szz - s.rho * s.uz * s.uz - r3;
+    s.p[3] = mo.sxy - s.rho * s.ux * s.uy;
+    s.p[4] = mo.sxz - s.rho * s.ux * s.uz;
+    s.p[5] = mo.syz - s.rho * s.uy * s.uz;
+    return s;
+}
+
 // ---------------------------------------------------------------- collision
 // shader.rs:815-847 (+ Guo forcing :442-455). On exit f holds the
 // post-collision f~ (before streaming), rho and u_out as the record saves.
 template <class T, bool FORCE, bool SPONGE, class Hook>
 __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z, T f[19], T &rho_out,
-                                        T u_out[3], Hook &&tau_hook) {
+                                        T u_out[3], Hook &&tau_hook, T ppost[6]) {
     const Mom<T> m = moments(f);
     const T rho = (T)1 + m.rho1;
     const T irho = rcp_(rho);
@@ -396,6 +431,44 @@ __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_
     u_out[0] = ux;
     u_out[1] = uy;
     u_out[2] = uz;
+    // non-equilibrium stress of the post-collision populations (what the
+    // next face_gather computes from them): (1 - 1/tau) Pi, plus the second
+    // moment of the Guo term (1 - 1/(2 tau)) (u_a G_b + u_b G_a). Exact
+    // without the absorbing layer; sponged lattices gather from f instead.
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ppost[k] = k1 * p[k];
+    if constexpr (FORCE) {
+        const T fo = (T)1 - (T)0.5 * om;
+        const T Gx = rho * gx, Gy = rho * gy, Gz = rho * gz;
+        ppost[0] = ppost[0] + fo * (T)2 * ux * Gx;
+        ppost[1] = ppost[1] + fo * (T)2 * uy * Gy;
+        ppost[2] = ppost[2] + fo * (T)2 * uz * Gz;
+        ppost[3] = ppost[3] + fo * (ux * Gy + uy * Gx);
+        ppost[4] = ppost[4] + fo * (ux * Gz + uz * Gx);
+        ppost[5] = ppost[5] + fo * (uy * Gz + uz * Gy);
+    }
+}
+
+// the face-source state after collide() (rho, u unchanged by the collision
+// unless the absorbing layer acts)
+template <class T, bool SPONGE>
+__device__ __forceinline__ FaceSrc<T> post_state(const DevLattice &L, const T f[19], T rho, const T u[3], const T ppost[6], bool src) {
+    FaceSrc<T> s;
+    if constexpr (SPONGE) {
+        if (src) return face_source(L, f);
+        s.rho = (T)1; s.ux = s.uy = s.uz = (T)0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s.p[k] = (T)0;
+        return s;
+    } else {
+        s.rho = rho;
+        s.ux = u[0];
+        s.uy = u[1];
+        s.uz = u[2];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s.p[k] = ppost[k];
+        return s;
+    }
 }
 
 struct NoHook {
@@ -486,18 +559,209 @@ __device__ __forceinline__ void block_sum6(double v[6], double (*sh)[6], uint32_
     }
 }
 
-// ---------------------------------------------------------------- wall blocks
-// One thread per wall record: Bouzidi / moving-wall links with momentum
-// exchange (shader.rs:401-436, 808-814), the collision, the record save
-// (:848-860), then the FP64 force/torque reduction of all records.
+// ---------------------------------------------------------------- faces
+// face_gather + face_write (shader.rs:630-687). A face cell's outgoing
+// populations are rebuilt from the post-collision state of its inner
+// neighbour m (clamped on the non-periodic axes): rho, u and Pi_neq, with
+// the inlet/outlet rules applied. Only the populations some colliding cell
+// reads are written: direction i of face cell b is live when b + c_i is an
+// interior cell (1..n-2 on every non-periodic axis). The other slots the
+// reference also writes are read by nobody.
+// The update for step k+1 is done by m right after its own collision in
+// step k: the live face slots are read and written by no other thread of
+// step k. Stores of face cells round with the reference's per-store hash
+// (key = cell * 19 + slot direction, shader.rs:510-512).
+
+// velocity set for a direction index known only at run time (2-bit fields)
+constexpr uint64_t pack_c(const int (&c)[19]) {
+    uint64_t v = 0;
+    for (int i = 0; i < 19; ++i) v |= (uint64_t)(c[i] + 1) << (2 * i);
+    return v;
+}
+constexpr uint64_t PCX = pack_c(CX), PCY = pack_c(CY), PCZ = pack_c(CZ);
+__device__ __forceinline__ int rcx(int i) { return (int)((PCX >> (2 * i)) & 3u) - 1; }
+__device__ __forceinline__ int rcy(int i) { return (int)((PCY >> (2 * i)) & 3u) - 1; }
+__device__ __forceinline__ int rcz(int i) { return (int)((PCZ >> (2 * i)) & 3u) - 1; }
+__device__ __forceinline__ int ropp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+template <class T> __device__ __forceinline__ T rwgt(int i) {
+    return i == 0 ? (T)(1.0 / 3.0) : (i <= 6 ? (T)(1.0 / 18.0) : (T)(1.0 / 36.0));
+}
+
+// the face cell's macroscopic state from its source's (shader.rs:670-681)
+template <class T> __device__ __forceinline__ void face_rule(const DevLattice &L, uint32_t ty, FaceSrc<T> &s) {
+    if (ty == 2u) {
+        if (L.total_p) {
+            const T ui2 = (T)(L.u_inf[0] * L.u_inf[0] + L.u_inf[1] * L.u_inf[1] + L.u_inf[2] * L.u_inf[2]);
+            s.rho = div_((T)1 / (T)3 + (T)0.5 * ui2, (T)1 / (T)3 + (T)0.5 * (s.ux * s.ux + s.uy * s.uy + s.uz * s.uz));
+        } else {
+            s.ux = (T)L.u_inf[0];
+            s.uy = (T)L.u_inf[1];
+            s.uz = (T)L.u_inf[2];
+        }
+    } else {
+        s.rho = (T)1;
+    }
+}
+
+// feq_i(rho, u) - w_i + fneq_i(Pi) for one direction (shader.rs:367-396, 682-685)
+template <class T> __device__ __forceinline__ T face_value(int i, const FaceSrc<T> &s) {
+    const T cx = (T)rcx(i), cy = (T)rcy(i), cz = (T)rcz(i), w = rwgt<T>(i);
+    const T cu = cx * s.ux + cy * s.uy + cz * s.uz;
+    const T usq = (T)1.5 * (s.ux * s.ux + s.uy * s.uy + s.uz * s.uz);
+    const T rho1 = s.rho - (T)1;
+    const T fe = w * (rho1 + s.rho * ((T)3 * cu + (T)4.5 * cu * cu - usq));
+    const T tr = (s.p[0] + s.p[1] + s.p[2]) * ((T)1 / (T)3);
+    const T q = cx * cx * s.p[0] + cy * cy * s.p[1] + cz * cz * s.p[2] +
+                (T)2 * (cx * cy * s.p[3] + cx * cz * s.p[4] + cy * cz * s.p[5]);
+    return fe + (T)4.5 * w * (q - tr);
+}
+
+// is direction i of face cell (bx, by, bz) read by a colliding cell?
+__device__ __forceinline__ bool face_live(const DevLattice &L, int bx, int by, int bz, int i) {
+    const int tx = bx + rcx(i), ty = by + rcy(i), tz = bz + rcz(i);
+    return (L.per[0] || (tx >= 1 && tx <= (int)L.nx - 2)) && (L.per[1] || (ty >= 1 && ty <= (int)L.ny - 2)) &&
+           (L.per[2] || (tz >= 1 && tz <= (int)L.nz - 2));
+}
+
+// store direction i of face cell b into its output slot of parity Q
+// (store_prev_out, shader.rs:273-305), dither from the per-store hash
+template <class C, int Q>
+__device__ __forceinline__ void face_store(const DevLattice &L, int bx, int by, int bz, int i, typename C::T v, uint32_t seed) {
+    const uint32_t b = ((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx;
+    uint32_t cell, dir;
+    if (Q == 0 || i == 0) {
+        cell = b;
+        dir = (uint32_t)ropp(i);
+    } else {
+        int tx = bx + rcx(i), ty = by + rcy(i), tz = bz + rcz(i);
+        tx = tx < 0 ? tx + (int)L.nx : (tx >= (int)L.nx ? tx - (int)L.nx : tx);
+        ty = ty < 0 ? ty + (int)L.ny : (ty >= (int)L.ny ? ty - (int)L.ny : ty);
+        tz = tz < 0 ? tz + (int)L.nz : (tz >= (int)L.nz ? tz - (int)L.nz : tz);
+        cell = ((uint32_t)tz * L.ny + (uint32_t)ty) * L.nx + (uint32_t)tx;
+        dir = (uint32_t)i;
+    }
+    const uint32_t d16 = C::SR ? (cell_hash(b * 19u + dir, seed) >> 16) : 0u;
+    ((typename C::E *)L.pops + (size_t)dir * L.ncell)[cell] = enc<C>(v, d16);
+}
+
+// Thread-serial face update of one face cell (standalone kernel)
+template <class C, int Q>
+__device__ __forceinline__ void face_cell_write(const DevLattice &L, int bx, int by, int bz, FaceSrc<typename C::T> s, uint32_t seed) {
+    const uint32_t b = ((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx;
+    face_rule(L, L.flags[b] & 3u, s);
+    for (int i = 0; i < 19; ++i)
+        if (face_live(L, bx, by, bz, i)) face_store<C, Q>(L, bx, by, bz, i, face_value(i, s), seed);
+}
+
+// Does a colliding cell at (x, y, z) feed face cells? (it is the clamp of
+// some face cell: coordinate 1 or n-2 on a non-periodic axis)
+__device__ __forceinline__ bool is_face_source(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z) {
+    return (!L.per[0] && (x == 1 || x == L.nx - 2)) || (!L.per[1] && (y == 1 || y == L.ny - 2)) ||
+           (!L.per[2] && (z == 1 || z == L.nz - 2));
+}
+
+// Wave-cooperative face update after a step of parity Q: for every lane
+// that is a face source, each face cell it feeds is written by 19 lanes of
+// the wave, one direction each. All lanes of the wave must call this.
+template <class C, int Q>
+__device__ __forceinline__ void face_writes_wave(const DevLattice &L, bool src, uint32_t x, uint32_t y, uint32_t z,
+                                                 const FaceSrc<typename C::T> &s, uint32_t seed) {
+    using T = typename C::T;
+    uint64_t bal = __ballot(src);
+    const int lane = (int)__lane_id();
+    while (bal) {
+        const int l = __builtin_ctzll(bal);
+        bal &= bal - 1;
+        const int sx = __shfl((int)x, l), sy = __shfl((int)y, l), sz = __shfl((int)z, l);
+        FaceSrc<T> m;
+        m.rho = __shfl(s.rho, l);
+        m.ux = __shfl(s.ux, l);
+        m.uy = __shfl(s.uy, l);
+        m.uz = __shfl(s.uz, l);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) m.p[k] = __shfl(s.p[k], l);
+        const bool lo[3] = {!L.per[0] && sx == 1, !L.per[1] && sy == 1, !L.per[2] && sz == 1};
+        const bool hi[3] = {!L.per[0] && sx == (int)L.nx - 2, !L.per[1] && sy == (int)L.ny - 2, !L.per[2] && sz == (int)L.nz - 2};
+        for (int dz = -1; dz <= 1; ++dz) {
+            if ((dz < 0 && !lo[2]) || (dz > 0 && !hi[2])) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                if ((dy < 0 && !lo[1]) || (dy > 0 && !hi[1])) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if ((dx < 0 && !lo[0]) || (dx > 0 && !hi[0])) continue;
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    const int bx = sx + dx, by = sy + dy, bz = sz + dz;
+                    const uint32_t b = ((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx;
+                    FaceSrc<T> fb = m;
+                    face_rule(L, L.flags[b] & 3u, fb);
+                    if (lane < 19 && face_live(L, bx, by, bz, lane))
+                        face_store<C, Q>(L, bx, by, bz, lane, face_value(lane, fb), seed);
+                }
+            }
+        }
+    }
+}
+
+// Standalone face update before a step of parity P (the reference's
+// face_gather + face_write in module P): gathers the stored post-collision
+// state of each face cell's inner neighbour. Used at the start of every
+// `run`, and before every step where the update cannot be fused.
+template <class C, int P> __global__ void __launch_bounds__(256) k_faces(DevLattice L, uint32_t seed) {
+    using T = typename C::T;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L.nface) return;
+    const uint32_t b = L.faces[k];
+    uint32_t x, y, z;
+    coords(L, b, x, y, z);
+    uint32_t mx = x, my = y, mz = z;
+    if (!L.per[0]) mx = min(max(mx, 1u), L.nx - 2);
+    if (!L.per[1]) my = min(max(my, 1u), L.ny - 2);
+    if (!L.per[2]) mz = min(max(mz, 1u), L.nz - 2);
+    const uint32_t m = (mz * L.ny + my) * L.nx + mx;
+    FaceSrc<T> s;
+    if ((L.flags[m] & 3u) == 0) {
+        Nbr nm;
+        nm.make(L, mx, my, mz, m);
+        T f[19];
+        gather_post<C, 1 - P>(L, nm, f);
+        s = face_source(L, f);
+    } else {
+        s.rho = (T)1;
+        s.ux = (T)L.u_inf[0];
+        s.uy = (T)L.u_inf[1];
+        s.uz = (T)L.u_inf[2];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s.p[j] = (T)0;
+    }
+    face_cell_write<C, 1 - P>(L, (int)x, (int)y, (int)z, s, seed);
+}
+
+// ---------------------------------------------------------------- special cells
+// k_special processes, for a step of parity P, the wall-record cells, one
+// thread per record: Bouzidi / moving-wall links with momentum exchange
+// (shader.rs:401-436, 808-814), the collision, the record save (:848-860);
+// then the FP64 force/torque reduction of all records (wave shuffles -> LDS
+// -> per-block partial -> the last block to finish sums the partials in
+// block order: deterministic).
+// A cell that is the inner neighbour of face cells (here or in k_bulk) also
+// rebuilds their outgoing populations for the next step from its own
+// post-collision state, which is what the next step's face_gather would
+// read: the face-cell slots it writes are touched by no other thread of
+// this step.
 template <class C, int P, bool FORCE, bool SPONGE>
-__device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a, uint32_t wb, uint32_t tid, uint32_t bs) {
+__global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
     using T = typename C::T;
     __shared__ double sh[16][6];
     __shared__ int last;
-    if (wb >= L.wall_blocks) return;   // whole block
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t seed_next = a.seed + 2654435761u;   // lattice.rs:676 seed of the next step
+    if (blockIdx.x >= L.wall_blocks) return;
+    const uint32_t wb = blockIdx.x;
     const uint32_t r = wb * bs + tid;
     T F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+    T pp[6];
+    bool src = false;
+    FaceSrc<T> fsrc;
+    uint32_t sx = 0, sy = 0, sz = 0;
     T *rfpost = (T *)L.rfpost;
     T *rmac = (T *)L.rmac;
     T *rft = (T *)L.rft;
@@ -505,7 +769,7 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
     if (r < L.nrec) {
         const uint32_t n = L.rcell[r];
         const uint32_t fg = L.flags[n];
-        if ((fg & 3u) == 0 && (fg >> 2) == r + 1) {
+        if ((fg & 3u) == 0 && rec_of(fg) == r + 1) {
             uint32_t x, y, z;
             coords(L, n, x, y, z);
             Nbr nb;
@@ -516,6 +780,24 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
             const T pcx = (T)x - (T)L.center[0], pcy = (T)y - (T)L.center[1], pcz = (T)z - (T)L.center[2];
             const T omx = (T)a.omega[0], omy = (T)a.omega[1], omz = (T)a.omega[2];
             if (mask) {
+                // issue every record load this cell needs before using any, so
+                // they are in flight together (the link loop would otherwise
+                // wait for each in turn)
+                uint32_t need = 0;
+                sfor<18>([&](auto I) {
+                    constexpr int i = I + 1;
+                    if (mask & (1u << i)) need |= (1u << i) | (1u << OPP[i]);
+                });
+                T fp[19];
+                uint32_t qw[9];
+                sfor<18>([&](auto I) {
+                    constexpr int i = I + 1;
+                    fp[i] = (need & (1u << i)) ? rfpost[(size_t)(i - 1) * NR + r] : (T)0;
+                });
+                sfor<9>([&](auto K) {
+                    constexpr int k = K;
+                    qw[k] = (mask & (3u << (2 * k + 1))) ? L.rq[(size_t)k * NR + r] : 0u;
+                });
                 T fin[19];
 #pragma unroll
                 for (int i = 0; i < 19; ++i) fin[i] = f[i];
@@ -524,9 +806,10 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
                     constexpr int o = OPP[i];
                     if (mask & (1u << i)) {
                         const T wi = wgt<T>(i);
-                        const T q = fmax((T)q_of(L, r, i), (T)0.001);
-                        const T fpi = rfpost[(size_t)(i - 1) * NR + r] + wi;
-                        const T fpo = rfpost[(size_t)(o - 1) * NR + r] + wi;
+                        const uint16_t qh = (uint16_t)(((i - 1) & 1) ? (qw[(i - 1) >> 1] >> 16) : (qw[(i - 1) >> 1] & 0xffffu));
+                        const T q = fmax((T)(float)__builtin_bit_cast(_Float16, qh), (T)0.001);
+                        const T fpi = fp[i] + wi;
+                        const T fpo = fp[o] + wi;
                         const T cx = (T)CX[i], cy = (T)CY[i], cz = (T)CZ[i];
                         const T xwx = pcx + q * cx, xwy = pcy + q * cy, xwz = pcz + q * cz;
                         const T uwx = omy * xwz - omz * xwy, uwy = omz * xwx - omx * xwz, uwz = omx * xwy - omy * xwx;
@@ -554,7 +837,7 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
                 if (L.wall_model && mask != 0) return wall_tau<T>(L, r, mask, pcx, pcy, pcz, ux, uy, uz, tau, omf);
                 return tau;
             };
-            collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, hook);
+            collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, hook, pp);
 #pragma unroll
             for (int i = 1; i < 19; ++i) rfpost[(size_t)(i - 1) * NR + r] = f[i];
             rmac[r] = rho;
@@ -574,6 +857,11 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
                 rft[(3 + k) * NR + r] = Tq[k];
             }
             scatter_post<C, P, false>(L, nb, f, a.seed);
+            src = is_face_source(L, x, y, z);
+            fsrc = post_state<T, SPONGE>(L, f, rho, u, pp, src);
+            sx = x;
+            sy = y;
+            sz = z;
         } else {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -582,6 +870,7 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
             }
         }
     }
+    if (__ballot(src)) face_writes_wave<C, P>(L, src, sx, sy, sz, fsrc, seed_next);
     double v[6] = {(double)F[0], (double)F[1], (double)F[2], (double)Tq[0], (double)Tq[1], (double)Tq[2]};
     block_sum6(v, sh, tid, bs);
     if (tid == 0) {
@@ -607,93 +896,36 @@ __device__ __forceinline__ void wall_path(const DevLattice &L, const StepArgs &a
     }
 }
 
-// ---------------------------------------------------------------- stream-collide
-// Grid: (xblocks * yblocks, wall_rows + nz); block (bx, by). Rows
-// blockIdx.y < wall_rows are wall blocks; the rest are bulk tiles of `by`
-// x-rows (split in xblocks pieces when nx > bx) at one z.
+// ---------------------------------------------------------------- bulk
+// Every fluid cell with no wall record (face-source cells included). Grid:
+// (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
+// xblocks pieces when nx > bx) at one z, so the +-1 x-offsets of parity 1
+// stay inside the block and waves touch whole cache lines.
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(1024) k_stream_collide(DevLattice L, StepArgs a) {
+__global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     using T = typename C::T;
-    const uint32_t bs = blockDim.x * blockDim.y;
-    const uint32_t tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if (blockIdx.y < L.wall_rows) {
-        wall_path<C, P, FORCE, SPONGE>(L, a, blockIdx.y * gridDim.x + blockIdx.x, tid, bs);
-        return;
-    }
     const uint32_t xb = blockIdx.x % L.xblocks, yb = blockIdx.x / L.xblocks;
     const uint32_t x = xb * L.bx + threadIdx.x;
     const uint32_t y = yb * L.by + threadIdx.y;
-    const uint32_t z = blockIdx.y - L.wall_rows;
-    if (x >= L.nx || y >= L.ny) return;
+    const uint32_t z = blockIdx.y;
+    const bool inside = x < L.nx && y < L.ny;
     const uint32_t n = (z * L.ny + y) * L.nx + x;
-    if (L.flags[n] != 0u) return;   // solid, face or wall-record cell
-    Nbr nb;
-    nb.make(L, x, y, z, n);
-    T f[19];
-    gather_pre<C, P>(L, nb, f);
-    T rho, u[3];
-    collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{});
-    scatter_post<C, P, false>(L, nb, f, a.seed);
-}
-
-// ---------------------------------------------------------------- faces
-// face_gather + face_write fused (shader.rs:630-687). The gather reads only
-// the previous-step outputs of interior cells and the write only touches
-// face-cell output slots, so one thread can do both.
-template <class C, int P> __global__ void __launch_bounds__(256) k_faces(DevLattice L, uint32_t seed) {
-    using T = typename C::T;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= L.nface) return;
-    const uint32_t b = L.faces[k];
-    uint32_t x, y, z;
-    coords(L, b, x, y, z);
-    uint32_t mx = x, my = y, mz = z;
-    if (!L.per[0]) mx = min(max(mx, 1u), L.nx - 2);
-    if (!L.per[1]) my = min(max(my, 1u), L.ny - 2);
-    if (!L.per[2]) mz = min(max(mz, 1u), L.nz - 2);
-    const uint32_t m = (mz * L.ny + my) * L.nx + mx;
-    T rho = (T)1;
-    T ux = (T)L.u_inf[0], uy = (T)L.u_inf[1], uz = (T)L.u_inf[2];
-    T p[6] = {0, 0, 0, 0, 0, 0};
-    if ((L.flags[m] & 3u) == 0) {
-        Nbr nm;
-        nm.make(L, mx, my, mz, m);
-        T f[19];
-        gather_post<C, 1 - P>(L, nm, f);
-        const Mom<T> mo = moments(f);
-        rho = (T)1 + mo.rho1;
-        ux = div_(mo.jx, rho) - (T)0.5 * (T)L.grav[0];
-        uy = div_(mo.jy, rho) - (T)0.5 * (T)L.grav[1];
-        uz = div_(mo.jz, rho) - (T)0.5 * (T)L.grav[2];
-        const T r3 = div_(mo.rho1, (T)3);
-        p[0] = mo.sxx - rho * ux * ux - r3;
-        p[1] = mo.syy - rho * uy * uy - r3;
-        p[2] = mo.szz - rho * uz * uz - r3;
-        p[3] = mo.sxy - rho * ux * uy;
-        p[4] = mo.sxz - rho * ux * uz;
-        p[5] = mo.syz - rho * uy * uz;
+    const uint32_t fg = inside ? L.flags[n] : 1u;
+    // solid, face and wall-record cells are not ours
+    const bool mine = fg == 0u;
+    T f[19], rho = (T)1, u[3] = {0, 0, 0}, pp[6] = {0, 0, 0, 0, 0, 0};
+    if (mine) {
+        Nbr nb;
+        nb.make(L, x, y, z, n);
+        gather_pre<C, P>(L, nb, f);
+        collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
+        scatter_post<C, P, false>(L, nb, f, a.seed);
     }
-    const uint32_t ty = L.flags[b] & 3u;
-    if (ty == 2u) {
-        if (L.total_p) {
-            const T ui2 = (T)(L.u_inf[0] * L.u_inf[0] + L.u_inf[1] * L.u_inf[1] + L.u_inf[2] * L.u_inf[2]);
-            rho = div_((T)1 / (T)3 + (T)0.5 * ui2, (T)1 / (T)3 + (T)0.5 * (ux * ux + uy * uy + uz * uz));
-        } else {
-            ux = (T)L.u_inf[0];
-            uy = (T)L.u_inf[1];
-            uz = (T)L.u_inf[2];
-        }
-    } else {
-        rho = (T)1;
+    const bool src = mine && is_face_source(L, x, y, z);
+    if (__ballot(src)) {
+        const FaceSrc<T> s = post_state<T, SPONGE>(L, f, rho, u, pp, src);
+        face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
     }
-    T fe[19], fq[19];
-    feq_arr<T>(rho - (T)1, ux, uy, uz, fe);
-    fneq_arr<T>(p, fq);
-#pragma unroll
-    for (int i = 0; i < 19; ++i) fe[i] = fe[i] + fq[i];
-    Nbr nbb;
-    nbb.make(L, x, y, z, b);
-    scatter_post<C, 1 - P, true>(L, nbb, fe, seed);
 }
 
 // ---------------------------------------------------------------- init
@@ -713,7 +945,7 @@ __device__ __forceinline__ bool cell_macro(const DevLattice &L, uint32_t m, type
     using T = typename C::T;
     const uint32_t fg = L.flags[m];
     if (fg & 3u) return false;
-    const uint32_t rec = fg >> 2;
+    const uint32_t rec = rec_of(fg);
     if (rec != 0) {
         const T *rmac = (const T *)L.rmac;
         const size_t NR = L.nrec;
@@ -835,19 +1067,26 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_sample_slices
 // ---------------------------------------------------------------- dispatch helpers
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-template <class C, int P>
-hipError_t step_launch(const DevLattice &L, const StepArgs &a, hipStream_t s) {
-    dim3 grid(L.xblocks * L.yblocks, L.wall_rows + L.nz), block(L.bx, L.by);
-    const bool force = L.forcing != 0, sponge = L.sponge != 0;
-    if (!force && !sponge) k_stream_collide<C, P, false, false><<<grid, block, 0, s>>>(L, a);
-    else if (force && !sponge) k_stream_collide<C, P, true, false><<<grid, block, 0, s>>>(L, a);
-    else if (!force && sponge) k_stream_collide<C, P, false, true><<<grid, block, 0, s>>>(L, a);
-    else k_stream_collide<C, P, true, true><<<grid, block, 0, s>>>(L, a);
+template <class C, int P, bool FORCE, bool SPONGE>
+hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk) {
+    k_special<C, P, FORCE, SPONGE><<<L.wall_blocks, SPECIAL_BLOCK, 0, s_special>>>(L, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_bulk<C, P, FORCE, SPONGE><<<dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), 0, s_bulk>>>(L, a);
     return hipGetLastError();
 }
 
-template <class C> hipError_t step_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    return p == 0 ? step_launch<C, 0>(L, a, s) : step_launch<C, 1>(L, a, s);
+template <class C, int P>
+hipError_t step_launch(const DevLattice &L, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    const bool force = L.forcing != 0, sponge = L.sponge != 0;
+    if (!force && !sponge) return step_launch4<C, P, false, false>(L, a, s1, s2);
+    if (force && !sponge) return step_launch4<C, P, true, false>(L, a, s1, s2);
+    if (!force && sponge) return step_launch4<C, P, false, true>(L, a, s1, s2);
+    return step_launch4<C, P, true, true>(L, a, s1, s2);
+}
+
+template <class C> hipError_t step_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    return p == 0 ? step_launch<C, 0>(L, a, s1, s2) : step_launch<C, 1>(L, a, s1, s2);
 }
 
 }  // namespace
@@ -881,12 +1120,12 @@ hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk) {
     switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: return step_launch_p<CF32>(L, p, a, s);
-        case SIMLAB_STORAGE_FP16: return step_launch_p<CF16>(L, p, a, s);
-        case SIMLAB_STORAGE_BF16: return step_launch_p<CBF16>(L, p, a, s);
-        default: return step_launch_p<CF64>(L, p, a, s);
+        case SIMLAB_STORAGE_FP32: return step_launch_p<CF32>(L, p, a, s_special, s_bulk);
+        case SIMLAB_STORAGE_FP16: return step_launch_p<CF16>(L, p, a, s_special, s_bulk);
+        case SIMLAB_STORAGE_BF16: return step_launch_p<CBF16>(L, p, a, s_special, s_bulk);
+        default: return step_launch_p<CF64>(L, p, a, s_special, s_bulk);
     }
 }
 

@@ -104,10 +104,15 @@ struct simlab_lattice {
     uint32_t nrec = 0, nface = 0, npix = 0, nvert = 0;
     uint64_t memory_bytes = 0;
     bool moving = false;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;    // faces, bulk kernel, copies
+    hipStream_t stream2 = nullptr;   // special kernel, concurrent with the bulk
+    hipEvent_t fork = nullptr, join = nullptr;
+    uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
-    DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, partials, counter, dump, mass_part,
-        mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
+    DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, partials,
+        counter, dump, mass_part, mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
+    bool fused_faces = true;   // face update fused into the previous step
+    bool concurrent = true;    // k_special on a second stream beside k_bulk
     double *hist_host = nullptr;   // pinned, KMAX x HIST_STRIDE
     bool timing = false;
     std::vector<hipEvent_t> ev;    // 2 per step of a chunk + 2 around the chunk
@@ -115,10 +120,14 @@ struct simlab_lattice {
     uint64_t last_main_launches = 0;
 
     ~simlab_lattice() {
-        if (stream) hipStreamSynchronize(stream);
-        for (hipEvent_t e : ev) hipEventDestroy(e);
-        if (hist_host) hipHostFree(hist_host);
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (stream2) (void)hipStreamSynchronize(stream2);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (hist_host) (void)hipHostFree(hist_host);
+        if (stream2) (void)hipStreamDestroy(stream2);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -153,10 +162,7 @@ void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) 
     L.by = by;
     L.xblocks = (L.nx + bx - 1) / bx;
     L.yblocks = (L.ny + by - 1) / by;
-    const uint32_t bs = bx * by;
-    L.wall_blocks = std::max<uint32_t>(1, (L.nrec + bs - 1) / bs);
-    const uint32_t gx = L.xblocks * L.yblocks;
-    L.wall_rows = (L.wall_blocks + gx - 1) / gx;
+    L.wall_blocks = std::max<uint32_t>(1, (L.nrec + SPECIAL_BLOCK - 1) / SPECIAL_BLOCK);
 }
 
 uint32_t default_block_threads() {
@@ -343,6 +349,10 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         throw;
     }
     hip_check(hipStreamCreateWithFlags(&lat->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&lat->stream2, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&lat->fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&lat->join, hipEventDisableTiming), "hipEventCreate");
+    for (uint64_t c = 0; c < n; ++c) lat->nbulk += h_flags[c] == 0 ? 1 : 0;
     hip_check(hipHostMalloc((void **)&lat->hist_host, (size_t)KMAX * HIST_STRIDE * 8, hipHostMallocDefault), "hipHostMalloc");
     hipStream_t st = lat->stream;
     auto up = [&](DevBuf &b, const void *src, size_t bytes) {
@@ -410,6 +420,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.slicedef = lat->slicedef.as<int4>();
     L.sacc = lat->sacc.as<float>();
     choose_shape(L, default_block_threads(), 0);
+    if (const char *e = std::getenv("SIMLAB_CFD_SERIAL")) lat->concurrent = !(e[0] == '1');
     lat->partials.alloc((size_t)L.wall_blocks * 8 * 8, mem);
     L.partials = lat->partials.as<double>();
     lat->memory_bytes = mem;
@@ -450,10 +461,32 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             const int par = (int)(step % 2);
             StepArgs a;
             pack_step(s, step, (uint32_t)k, a);
-            hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
-            if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
-            hip_check(launch_stream_collide(lat->L, par, a, st), "stream-collide kernel");
-            if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+            if (done + k == 0 && lat->fused_faces) {
+                // first step of this call: rebuild this step's face inputs from
+                // the stored state, then the next step's (constant where the
+                // source cell is solid; overwritten by k_special otherwise). In
+                // this order: with parity 0 the next step's face slots are the
+                // face cells' own slots, which hold the source cells' parity-1
+                // outputs toward them that the first gather reads.
+                hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
+                hip_check(launch_faces(lat->L, 1 - par, (uint32_t)(step + 1) * 2654435761u, st), "face kernel");
+            } else if (!lat->fused_faces) {
+                hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
+            }
+            if (lat->concurrent) {
+                // k_special on the side stream, concurrent with k_bulk
+                hip_check(hipEventRecord(lat->fork, st), "event");
+                hip_check(hipStreamWaitEvent(lat->stream2, lat->fork, 0), "event wait");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
+                hip_check(launch_stream_collide(lat->L, par, a, lat->stream2, st), "stream-collide kernels");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+                hip_check(hipEventRecord(lat->join, lat->stream2), "event");
+                hip_check(hipStreamWaitEvent(st, lat->join, 0), "event wait");
+            } else {
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
+                hip_check(launch_stream_collide(lat->L, par, a, st, st), "stream-collide kernels");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+            }
             if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
         }
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
@@ -539,6 +572,7 @@ int simlab_cfd_lattice_info(const simlab_lattice *lat, simlab_lattice_info *out)
     out->storage = lat->storage;
     out->moving = lat->moving ? 1 : 0;
     out->tau0 = lat->L.tau0;
+    out->bulk_cells = lat->nbulk;
     return SIMLAB_OK;
 }
 

@@ -46,18 +46,36 @@ def test_gpu_info_names_the_device():
     assert "gfx9" in info
 
 
+def decode(bits, storage):
+    b = np.asarray(bits, np.int64).astype(np.uint16)
+    if storage == sa.Storage.FP16:
+        return b.view(np.float16).astype(np.float64) / 32768.0
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def live_post(pops, ora, p):
+    """Post-collision outputs of every colliding cell after a step of parity
+    p (the state the next step consumes); face-cell slots are excluded: the
+    library writes the next step's face inputs one step earlier."""
+    flags = ora.flags()
+    fluid = np.nonzero((flags & 3) == 0)[0]
+    _, st = O.aa_maps(ora.setup.dims, p)
+    return pops[st[fluid].astype(np.int64)]
+
+
 @pytest.mark.parametrize("storage", [sa.Storage.FP64, sa.Storage.FP32])
 def test_one_step_populations_match_oracle(storage):
     setup, geom, _ = small_channel(storage)
     say("case one_step", storage.name, setup.dims)
     lat, ora = both(setup, geom)
     assert lat.nrec == ora.nrec and lat.nface == ora.nface
-    p0g, p0o = lat.get_pops(), ora.get_pops()
-    assert np.array_equal(p0g, p0o) if storage == sa.Storage.FP64 else np.max(np.abs(p0g - p0o)) < 1e-9
+    p0g, p0o = lat.get_pops().astype(np.float64), ora.get_pops().astype(np.float64)
+    assert np.max(np.abs(p0g - p0o)) < (1e-16 if storage == sa.Storage.FP64 else 1e-8)   # init (FMA rounding)
     for k in range(2):   # both parities
         hg = lat.run_still(1)
         ho = ora.run_still(1)
-        pg, po = lat.get_pops().astype(np.float64), ora.get_pops().astype(np.float64)
+        pg = live_post(lat.get_pops().astype(np.float64), ora, k % 2)
+        po = live_post(ora.get_pops().astype(np.float64), ora, k % 2)
         tol = 1e-12 if storage == sa.Storage.FP64 else 2e-7
         d = np.max(np.abs(pg - po))
         say(f"  step {k + 1}: max |dpop| = {d:.3e}, F gpu {hg[0][:3]} oracle {ho[0][:3]}")
@@ -94,15 +112,23 @@ def test_half_storage_one_step_bits(storage):
     setup, geom, _ = small_channel(storage)
     say("case half_bits", storage.name, setup.dims)
     lat, ora = both(setup, geom)
-    assert np.array_equal(lat.get_pops(), ora.get_pops())   # init is exact
+    d0 = np.abs(lat.get_pops().astype(np.int64) - ora.get_pops().astype(np.int64))
+    assert d0.max() <= 1 and np.mean(d0 == 0) > 0.999   # init: FMA rounding may move a value by 1 ulp
     for k in range(2):
         lat.run_still(1)
         ora.run_still(1)
-        pg, po = lat.get_pops().astype(np.int64), ora.get_pops().astype(np.int64)
+        pg = live_post(lat.get_pops().astype(np.int64), ora, k % 2)
+        po = live_post(ora.get_pops().astype(np.int64), ora, k % 2)
         diff = np.abs(pg - po)
         frac = np.mean(diff == 0)
-        say(f"  step {k + 1}: identical {frac:.5f}, max diff {diff.max()} ulp")
-        assert frac > 0.99 and diff.max() <= 1
+        dv = np.abs(decode(pg, storage) - decode(po, storage))
+        say(f"  step {k + 1}: identical {frac:.5f}, max diff {diff.max()} ulp, max decoded diff {dv.max():.2e}")
+        # within one unit in the last place at the populations' scale
+        # (|f~| ~ 0.02): near zero an FP32 rounding difference spans more
+        # 16-bit ulps. Face-adjacent cells see face values the library
+        # computes from unrounded registers, the oracle from stored values.
+        ulp = 0.02 * (2.0 ** -10 if storage == sa.Storage.FP16 else 2.0 ** -7)
+        assert frac > 0.99 and dv.max() <= ulp
 
 
 @pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
@@ -142,7 +168,7 @@ def test_features_match_oracle():
     lat, ora = both(setup, geom)
     hg = lat.run_still(30)
     ho = ora.run_still(30)
-    pg, po = lat.get_pops(), ora.get_pops()
+    pg, po = live_post(lat.get_pops(), ora, 1), live_post(ora.get_pops(), ora, 1)
     d = np.max(np.abs(pg - po))
     say(f"  max |dpop| {d:.3e}, force rel {np.max(np.abs(hg - ho)) / np.max(np.abs(ho)):.3e}")
     assert d < 1e-11
@@ -166,16 +192,30 @@ def test_probes_match_oracle():
     assert np.all(np.isnan(lat.vertex_means()))
 
 
-def test_mass_is_conserved_in_a_periodic_box():
-    setup, geom = W.ball_box(24, 4.0, 0.05, 0.03, 0.0, [sa.FaceBc.PERIODIC] * 6, storage=sa.Storage.FP64)
+def test_mass_is_conserved_with_halfway_walls():
+    # halfway bounce-back (channel walls at y, z = 0.5 and n - 1.5) returns
+    # every population that meets the wall: total mass is conserved to
+    # round-off in a box periodic in x (Bouzidi links with q != 1/2 are not
+    # conservative, so no sphere here)
+    dims = (24, 16, 12)
+    setup = sa.LatticeSetup(dims=dims, faces=(sa.FaceBc.PERIODIC,) * 6, nu=0.05, u_init=(0.03, 0.0, 0.0),
+                            body_force=(1e-6, 0.0, 0.0), storage=sa.Storage.FP64, debug_dump=True)
+    geom = W.static_geometry(merge(*W.channel_walls(dims)), dims)
     say("case mass FP64")
     lat, ora = both(setup, geom)
-    m0 = lat.mass()
-    assert m0 == pytest.approx(ora.mass(), rel=1e-14)
-    lat.run_still(100)
+    assert lat.mass() == pytest.approx(ora.mass(), rel=1e-14)
+    # the first step carries a one-time change (the records start from
+    # feq(u_init) as the previous step's outputs, the populations from the
+    # init layout), identical in the oracle; from then on mass is conserved
+    lat.run_still(1)
+    ora.run_still(1)
     m1 = lat.mass()
-    say(f"  mass {m0:.15g} -> {m1:.15g}")
-    assert abs(m1 - m0) / m0 < 1e-12
+    lat.run_still(100)
+    ora.run_still(100)
+    m2 = lat.mass()
+    say(f"  mass {m1:.15g} -> {m2:.15g} (oracle {ora.mass():.15g})")
+    assert abs(m2 - m1) / m1 < 1e-12
+    assert m2 == pytest.approx(ora.mass(), rel=1e-13)
 
 
 def test_macros_needs_debug_dump():

@@ -173,3 +173,20 @@ def test_sround_is_unbiased_and_within_one_ulp():
     d = np.array([O.half_to_float(O.enc_f16_sr(x / 32768.0, int(k))) for k in range(0, 65536, 13)])
     assert np.all(np.abs(d - x) <= ulp)
     assert abs(d.mean() - x) < 0.01 * ulp
+
+
+def test_face_update_one_step_early_is_the_same_schedule():
+    # The GPU library rebuilds the face cells' inputs for step k+1 right
+    # after step k (from the source cells' fresh outputs) instead of at the
+    # start of step k+1, and at the start of each run does this step's face
+    # update before the next step's. Run in the oracle, that schedule gives
+    # bit-identical results to the reference's order.
+    setup, geom, meta = W.sphere_channel((40, 20, 16), 3.5, 12.0, 40.0, 0.05, 0.0, sa.Storage.FP64, debug_dump=True,
+                                         subdiv=3)
+    a = O.OracleLattice(setup, geom, store=O.F64)
+    b = O.OracleLattice(setup, geom, store=O.F64)
+    b.set_fused(True)
+    ha = a.run_still(60)
+    hb = np.concatenate([b.run_still(25), b.run_still(35)])
+    assert np.array_equal(ha, hb)
+    assert np.array_equal(a.macros_f64(), b.macros_f64(), equal_nan=True)
