@@ -153,8 +153,11 @@ typedef struct simlab_lattice_info {
     int32_t storage;             /* effective storage                       */
     int32_t moving;
     double tau0;
-    uint64_t bulk_cells;         /* cells of the plain (no wall, no face
-                                    duty) stream-collide kernel            */
+    uint64_t bulk_cells;         /* cells the bulk stream-collide kernel
+                                    updates                                */
+    uint64_t halfway_cells;      /* wall cells with only halfway (q = 1/2)
+                                    links, bounced back without a record;
+                                    nrec counts the other wall cells       */
 } simlab_lattice_info;
 
 typedef struct simlab_lattice simlab_lattice;            /* opaque */

@@ -913,17 +913,20 @@ template <class T> Lat<T> *make_lat(const simlab_lattice_setup *su, const simlab
     std::vector<std::pair<uint32_t, uint32_t>> rec_of;
     double ui[3] = {su->u_init[0], su->u_init[1], su->u_init[2]};
     double usq = 1.5 * (ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
-    float fe_sh[19];   // feq_shifted (lattice.rs:191-200), computed in f64, stored f32
+    // feq_shifted (lattice.rs:191-200), computed in f64 and stored as f32
+    // words; the FP64 storage keeps the f64 values (no f32 record words)
+    T fe_sh[19];
     for (int i = 0; i < 19; ++i) {
         double cu = CV[i][0] * ui[0] + CV[i][1] * ui[1] + CV[i][2] * ui[2];
-        fe_sh[i] = (float)(weight_d(i) * (3.0 * cu + 4.5 * cu * cu - usq));
+        double v = weight_d(i) * (3.0 * cu + 4.5 * cu * cu - usq);
+        fe_sh[i] = store == ST_F64 ? (T)v : (T)(float)v;
     }
     auto init_rec = [&](Rec<T> &R, uint32_t cell, uint32_t mask, const float *q) {
         R.mask = mask; R.cell = cell;
         for (int k = 0; k < 9; ++k) R.q[k] = (uint32_t)f16_bits_host(q[2 * k + 1]) | ((uint32_t)f16_bits_host(q[2 * k + 2]) << 16);
-        for (int i = 1; i < 19; ++i) R.fpost[i - 1] = (T)fe_sh[i];
-        R.mac[0] = (T)1.0f;
-        for (int k = 0; k < 3; ++k) R.mac[1 + k] = (T)(float)ui[k];
+        for (int i = 1; i < 19; ++i) R.fpost[i - 1] = fe_sh[i];
+        R.mac[0] = (T)1.0;
+        for (int k = 0; k < 3; ++k) R.mac[1 + k] = store == ST_F64 ? (T)ui[k] : (T)(float)ui[k];
         for (int k = 0; k < 6; ++k) R.ft[k] = 0;
     };
     std::vector<uint32_t> solid;

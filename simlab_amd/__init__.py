@@ -83,6 +83,7 @@ class _Info(C.Structure):
         ("steps_done", C.c_uint64), ("cells", C.c_uint64), ("memory_bytes", C.c_uint64), ("dims", C.c_uint32 * 3),
         ("nrec", C.c_uint32), ("nface", C.c_uint32), ("npix", C.c_uint32), ("nvert", C.c_uint32),
         ("storage", C.c_int32), ("moving", C.c_int32), ("tau0", C.c_double), ("bulk_cells", C.c_uint64),
+        ("halfway_cells", C.c_uint64),
     ]
 
 
@@ -424,6 +425,10 @@ class Lattice:
     @property
     def memory_bytes(self) -> int:
         return int(self._info().memory_bytes)
+
+    @property
+    def halfway_cells(self) -> int:
+        return int(self._info().halfway_cells)
 
     @property
     def bulk_cells(self) -> int:

@@ -12,6 +12,10 @@ namespace simlab {
 constexpr uint32_t KMAX = 512;   // steps per device-to-host force copy (lattice.rs:134)
 constexpr uint32_t HIST_STRIDE = 8;   // doubles per step in the force history
 constexpr uint32_t SPECIAL_BLOCK = 256;
+// flags bit 31: a wall cell whose cut links are all halfway (q = 1/2) and
+// handled without a record; bits 2-19 then hold its link mask (direction i
+// at bit i + 1). Otherwise bits 2-30 hold the record index + 1.
+constexpr uint32_t FLAG_HALFWAY = 1u << 31;
 
 // Device view of one lattice. Population slot of direction i and cell n is
 // element i * ncell + n of `pops` (SoA, x fastest, as shader.rs:147-152).
@@ -38,8 +42,10 @@ struct DevLattice {
 
 
     double *hist;                // [KMAX][HIST_STRIDE]
-    double *partials;            // [wall_blocks][8]
-    uint32_t *counter;           // last-block detection, returns to 0 after each step
+    uint32_t nhw;                // halfway wall cells (no record)
+    uint32_t *hw_cells;          // [nhw], sorted
+    uint32_t hw_blocks;          // k_special blocks for them
+    double *partials;            // [KMAX][wall_blocks + hw_blocks][8] per-block F, T of each step of a chunk
 
     // physics (lattice units)
     double tau0, smag2, sponge_tau, sigma;
@@ -94,6 +100,8 @@ hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+// force history of the first nsteps steps of a chunk from the partials
+hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t s);
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
 
 }  // namespace simlab

@@ -39,10 +39,10 @@ constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -
 constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
 
 // flags word: bits 0-1 cell type (0 fluid, 1 solid, 2 velocity face, 3 open
-// face), bits 2-30 wall record index + 1
+// face), bits 2-30 wall record index + 1, or FLAG_HALFWAY with a link mask
 constexpr uint32_t REC_MASK = 0x1FFFFFFFu;
 // wall record index + 1 of a flags word (0: none)
-__device__ __forceinline__ uint32_t rec_of(uint32_t fg) { return (fg >> 2) & REC_MASK; }
+__device__ __forceinline__ uint32_t rec_of(uint32_t fg) { return (fg & FLAG_HALFWAY) ? 0u : ((fg >> 2) & REC_MASK); }
 
 template <class T> __device__ __forceinline__ constexpr T wgt(int i) {
     return i == 0 ? (T)(1.0 / 3.0) : (i <= 6 ? (T)(1.0 / 18.0) : (T)(1.0 / 36.0));
@@ -735,9 +735,37 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_faces(DevLatt
     face_cell_write<C, 1 - P>(L, (int)x, (int)y, (int)z, s, seed);
 }
 
+// Block sum of F, T into the block's partial slot of this step; the sums
+// over blocks are formed after the chunk of steps by k_reduce_hist (in block
+// order: deterministic), so no fence or atomic is needed here.
+__device__ __forceinline__ void reduce_ft(const DevLattice &L, const StepArgs &a, double v[6], double (*sh)[6],
+                                          uint32_t tid, uint32_t bs, uint32_t nblocks) {
+    block_sum6(v, sh, tid, bs);
+    if (tid == 0) {
+        double *o = &L.partials[((size_t)a.slot * nblocks + blockIdx.x) * 8];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) o[j] = v[j];
+    }
+}
+
+// force history of a chunk: block s sums the partials of step s
+__global__ void __launch_bounds__(256) k_reduce_hist(const double *partials, uint32_t nblocks, double *hist) {
+    __shared__ double sh[16][6];
+    const uint32_t s = blockIdx.x, tid = threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t b = tid; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v[j] += partials[((size_t)s * nblocks + b) * 8 + j];
+    block_sum6(v, sh, tid, blockDim.x);
+    if (tid == 0)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) hist[(size_t)s * HIST_STRIDE + j] = v[j];
+}
+
 // ---------------------------------------------------------------- special cells
 // k_special processes, for a step of parity P, the wall-record cells, one
-// thread per record: Bouzidi / moving-wall links with momentum exchange
+// thread per record (blocks [0, wall_blocks)), and the momentum exchange of
+// the halfway wall cells (the blocks after those): Bouzidi / moving-wall links with momentum exchange
 // (shader.rs:401-436, 808-814), the collision, the record save (:848-860);
 // then the FP64 force/torque reduction of all records (wave shuffles -> LDS
 // -> per-block partial -> the last block to finish sums the partials in
@@ -751,10 +779,28 @@ template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
     using T = typename C::T;
     __shared__ double sh[16][6];
-    __shared__ int last;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t seed_next = a.seed + 2654435761u;   // lattice.rs:676 seed of the next step
-    if (blockIdx.x >= L.wall_blocks) return;
+    const uint32_t nblocks = L.wall_blocks + L.hw_blocks;
+    if (blockIdx.x >= L.wall_blocks) {
+        // momentum exchange of the halfway wall cells (their bounce-back is
+        // done by k_bulk): reads only last step's outputs, which no thread
+        // of this step writes
+        const uint32_t j = (blockIdx.x - L.wall_blocks) * bs + tid;
+        T F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+        if (j < L.nhw) {
+            const uint32_t n = L.hw_cells[j];
+            const uint32_t fg = L.flags[n];
+            uint32_t x, y, z;
+            coords(L, n, x, y, z);
+            Nbr nb;
+            nb.make(L, x, y, z, n);
+            halfway_force<C, P>(L, a, nb, (fg >> 1) & 0x7FFFEu, F, Tq);
+        }
+        double v[6] = {(double)F[0], (double)F[1], (double)F[2], (double)Tq[0], (double)Tq[1], (double)Tq[2]};
+        reduce_ft(L, a, v, sh, tid, bs, nblocks);
+        return;
+    }
     const uint32_t wb = blockIdx.x;
     const uint32_t r = wb * bs + tid;
     T F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
@@ -872,35 +918,92 @@ __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
     }
     if (__ballot(src)) face_writes_wave<C, P>(L, src, sx, sy, sz, fsrc, seed_next);
     double v[6] = {(double)F[0], (double)F[1], (double)F[2], (double)Tq[0], (double)Tq[1], (double)Tq[2]};
-    block_sum6(v, sh, tid, bs);
-    if (tid == 0) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) __hip_atomic_store(&L.partials[(size_t)wb * 8 + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const uint32_t prev = atomicAdd(L.counter, 1u);
-        last = (prev == L.wall_blocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t b = tid; b < L.wall_blocks; b += bs)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) s[j] += __hip_atomic_load(&L.partials[(size_t)b * 8 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    block_sum6(s, sh, tid, bs);
-    if (tid == 0) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) L.hist[(size_t)a.slot * HIST_STRIDE + j] = s[j];
-        __hip_atomic_store(L.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    reduce_ft(L, a, v, sh, tid, bs, nblocks);
 }
 
 // ---------------------------------------------------------------- bulk
-// Every fluid cell with no wall record (face-source cells included). Grid:
+// Every fluid cell with no wall record (face sources and halfway wall cells
+// included). Grid:
 // (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
 // xblocks pieces when nx > bx) at one z, so the +-1 x-offsets of parity 1
 // stay inside the block and waves touch whole cache lines.
+// Halfway bounce-back without a record (cells flagged FLAG_HALFWAY): the
+// population a cell sent toward the wall last step is read back from the
+// slot it was written to (load_prev_out for direction i: (n, opp i) after a
+// parity-0 step, (n + c_i, i) after a parity-1 step). The wall neighbour is
+// not a face cell and does not collide, so no other thread touches that slot
+// in this step. With q = 1/2 the Bouzidi rule (shader.rs:419) reduces to
+// f_opp = f_post,i - 6 w (c . u_w), and the momentum exchange is the same
+// (c - u_w) f_post,i + (c + u_w) f_b (shader.rs:423-424).
+// wall velocity (omega x x_w) at the halfway point of link i, and c . u_w
+template <class T> struct HalfwayLink {
+    T cx, cy, cz, xwx, xwy, xwz, uwx, uwy, uwz, cu;
+};
+template <int i, class T>
+__device__ __forceinline__ HalfwayLink<T> halfway_link(const DevLattice &L, const StepArgs &a, const Nbr &nb) {
+    HalfwayLink<T> h;
+    h.cx = (T)CX[i];
+    h.cy = (T)CY[i];
+    h.cz = (T)CZ[i];
+    h.xwx = (T)nb.x - (T)L.center[0] + (T)0.5 * h.cx;
+    h.xwy = (T)nb.y - (T)L.center[1] + (T)0.5 * h.cy;
+    h.xwz = (T)nb.z - (T)L.center[2] + (T)0.5 * h.cz;
+    const T omx = (T)a.omega[0], omy = (T)a.omega[1], omz = (T)a.omega[2];
+    h.uwx = omy * h.xwz - omz * h.xwy;
+    h.uwy = omz * h.xwx - omx * h.xwz;
+    h.uwz = omx * h.xwy - omy * h.xwx;
+    h.cu = h.cx * h.uwx + h.cy * h.uwy + h.cz * h.uwz;
+    return h;
+}
+
+// f_post,i of the previous step, read from where it was stored
+template <class C, int P, int i>
+__device__ __forceinline__ typename C::T halfway_fpi(const DevLattice &L, const Nbr &nb) {
+    if constexpr (P == 0) return dec<C>(popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()]);
+    else return dec<C>(popdir<C>(L, OPP[i])[nb.n]);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
+                                              typename C::T f[19]) {
+    using T = typename C::T;
+    sfor<18>([&](auto I) {
+        constexpr int i = I + 1;
+        if (m & (1u << i)) {
+            const T wi = wgt<T>(i);
+            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
+            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            f[OPP[i]] = (fpi - (T)6 * wi * h.cu) - wi;
+        }
+    });
+}
+
+// the momentum exchange of those links (the slots read above are unchanged
+// by this step's own stores)
+template <class C, int P>
+__device__ __forceinline__ void halfway_force(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
+                                              typename C::T F[3], typename C::T Tq[3]) {
+    using T = typename C::T;
+    sfor<18>([&](auto I) {
+        constexpr int i = I + 1;
+        if (m & (1u << i)) {
+            const T wi = wgt<T>(i);
+            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
+            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            const T fb = fpi - (T)6 * wi * h.cu;
+            const T flx = (h.cx - h.uwx) * fpi + (h.cx + h.uwx) * fb;
+            const T fly = (h.cy - h.uwy) * fpi + (h.cy + h.uwy) * fb;
+            const T flz = (h.cz - h.uwz) * fpi + (h.cz + h.uwz) * fb;
+            F[0] = F[0] + flx;
+            F[1] = F[1] + fly;
+            F[2] = F[2] + flz;
+            Tq[0] = Tq[0] + (h.xwy * flz - h.xwz * fly);
+            Tq[1] = Tq[1] + (h.xwz * flx - h.xwx * flz);
+            Tq[2] = Tq[2] + (h.xwx * fly - h.xwy * flx);
+        }
+    });
+}
+
 template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     using T = typename C::T;
@@ -911,19 +1014,24 @@ __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     const bool inside = x < L.nx && y < L.ny;
     const uint32_t n = (z * L.ny + y) * L.nx + x;
     const uint32_t fg = inside ? L.flags[n] : 1u;
-    // solid, face and wall-record cells are not ours
-    const bool mine = fg == 0u;
-    T f[19], rho = (T)1, u[3] = {0, 0, 0}, pp[6] = {0, 0, 0, 0, 0, 0};
+    // ours: fluid cells without a record (plain or halfway wall cells);
+    // solid, face and wall-record cells are not
+    const bool mine = (fg & 3u) == 0 && rec_of(fg) == 0;
+    const uint32_t hwm = (fg & FLAG_HALFWAY) ? ((fg >> 1) & 0x7FFFEu) : 0u;
+    T f[19];
     if (mine) {
         Nbr nb;
         nb.make(L, x, y, z, n);
         gather_pre<C, P>(L, nb, f);
+        if (hwm) halfway_links<C, P>(L, a, nb, hwm, f);
+        T rho, u[3], pp[6];
         collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
         scatter_post<C, P, false>(L, nb, f, a.seed);
     }
     const bool src = mine && is_face_source(L, x, y, z);
     if (__ballot(src)) {
-        const FaceSrc<T> s = post_state<T, SPONGE>(L, f, rho, u, pp, src);
+        FaceSrc<T> s;
+        if (src) s = face_source(L, f);
         face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
     }
 }
@@ -1069,7 +1177,7 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 
 template <class C, int P, bool FORCE, bool SPONGE>
 hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk) {
-    k_special<C, P, FORCE, SPONGE><<<L.wall_blocks, SPECIAL_BLOCK, 0, s_special>>>(L, a);
+    k_special<C, P, FORCE, SPONGE><<<L.wall_blocks + L.hw_blocks, SPECIAL_BLOCK, 0, s_special>>>(L, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_bulk<C, P, FORCE, SPONGE><<<dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), 0, s_bulk>>>(L, a);
@@ -1162,6 +1270,11 @@ hipError_t launch_mass(const DevLattice &L, int p, double *partials, uint32_t nb
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_sum_final<<<1, 256, 0, s>>>(partials, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t s) {
+    k_reduce_hist<<<nsteps, 256, 0, s>>>(L.partials, L.wall_blocks + L.hw_blocks, L.hist);
     return hipGetLastError();
 }
 

@@ -102,6 +102,7 @@ struct simlab_lattice {
     uint64_t ncell = 0;
     uint64_t steps_done = 0;
     uint32_t nrec = 0, nface = 0, npix = 0, nvert = 0;
+    uint64_t nhalfway = 0;           // wall cells handled without a record
     uint64_t memory_bytes = 0;
     bool moving = false;
     hipStream_t stream = nullptr;    // faces, bulk kernel, copies
@@ -109,7 +110,7 @@ struct simlab_lattice {
     hipEvent_t fork = nullptr, join = nullptr;
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
-    DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, partials,
+    DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
         counter, dump, mass_part, mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
     bool fused_faces = true;   // face update fused into the previous step
     bool concurrent = true;    // k_special on a second stream beside k_bulk
@@ -246,8 +247,53 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         }
     lat->nface = (uint32_t)faces.size();
 
-    // wall records, SoA (record_words, lattice.rs:202-218)
-    const uint32_t nrec = geom_kind == SIMLAB_GEOM_STATIC ? (uint32_t)g->nwall : 0u;
+    // flags: face type, then solid (lattice.rs:489-526)
+    std::vector<uint32_t> h_flags(n, 0);
+    for (uint32_t f : faces) {
+        uint32_t z = f / (nx * ny), r = f - z * nx * ny, y = r / nx, x = r - y * nx;
+        h_flags[f] = (uint32_t)face_type(x, y, z);
+    }
+    if (geom_kind == SIMLAB_GEOM_STATIC)
+        for (uint64_t k = 0; k < g->nsolid; ++k)
+            if (h_flags[g->solid[k]] == 0) h_flags[g->solid[k]] = 1;
+
+    // Wall cells whose cut links all sit halfway (q = 1/2) and lead into
+    // cells that are not face cells need no record: the bulk kernel bounces
+    // them back from the slot the cell's own last output went to (static
+    // geometry, no wall model, no sponge). Every other wall cell keeps a
+    // record (record_words, lattice.rs:202-218), SoA.
+    bool hw_ok = geom_kind == SIMLAB_GEOM_STATIC && !su->wall_model;
+    for (int k = 0; k < 6; ++k)
+        if (su->sponge[k]) hw_ok = false;
+    if (const char *e = std::getenv("SIMLAB_CFD_HALFWAY")) hw_ok = hw_ok && e[0] != '0';
+    std::vector<uint32_t> keep;
+    uint64_t nhalfway = 0;
+    if (geom_kind == SIMLAB_GEOM_STATIC) {
+        for (uint64_t r = 0; r < g->nwall; ++r) {
+            const simlab_wall_cell &w = g->walls[r];
+            const uint32_t c = w.cell;
+            bool hw = hw_ok && (h_flags[c] & 3u) == 0;
+            if (hw) {
+                const uint32_t cz = c / (nx * ny), rr = c - cz * nx * ny, cy = rr / nx, cx = rr - cy * nx;
+                for (int i = 1; i < 19 && hw; ++i) {
+                    if (!(w.mask & (1u << i))) continue;
+                    if (f16_bits_host(w.q[i]) != 0x3800) hw = false;   // q != 1/2
+                    const uint32_t tx = (uint32_t)(((int)cx + CV[i][0] + (int)nx) % (int)nx);
+                    const uint32_t ty = (uint32_t)(((int)cy + CV[i][1] + (int)ny) % (int)ny);
+                    const uint32_t tz = (uint32_t)(((int)cz + CV[i][2] + (int)nz) % (int)nz);
+                    if (face_type(tx, ty, tz) >= 0) hw = false;
+                }
+            }
+            if (hw) {
+                h_flags[c] = FLAG_HALFWAY | ((w.mask & 0x7FFFEu) << 1);
+                ++nhalfway;
+            } else {
+                keep.push_back((uint32_t)r);
+            }
+        }
+    }
+    lat->nhalfway = nhalfway;
+    const uint32_t nrec = (uint32_t)keep.size();
     lat->nrec = nrec;
     const size_t rb = real_bytes(su->storage);
     std::vector<uint32_t> h_mask(nrec), h_cell(nrec), h_q((size_t)9 * nrec);
@@ -255,42 +301,29 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     {
         const double *ui = su->u_init;
         const double usq = 1.5 * (ui[0] * ui[0] + ui[1] * ui[1] + ui[2] * ui[2]);
-        float fe[19];   // feq_shifted (lattice.rs:191-200)
+        double fe[19];   // feq_shifted (lattice.rs:191-200); f32 words except in FP64 storage
         for (int i = 0; i < 19; ++i) {
             const double cu = CV[i][0] * ui[0] + CV[i][1] * ui[1] + CV[i][2] * ui[2];
-            fe[i] = (float)(weight(i) * (3.0 * cu + 4.5 * cu * cu - usq));
+            fe[i] = weight(i) * (3.0 * cu + 4.5 * cu * cu - usq);
         }
-        auto put = [&](std::vector<unsigned char> &v, size_t idx, float x) {
+        auto put = [&](std::vector<unsigned char> &v, size_t idx, double x) {
             if (rb == 8) {
-                double d = (double)x;
-                std::memcpy(&v[idx * 8], &d, 8);
+                std::memcpy(&v[idx * 8], &x, 8);
             } else {
-                std::memcpy(&v[idx * 4], &x, 4);
+                const float f = (float)x;
+                std::memcpy(&v[idx * 4], &f, 4);
             }
         };
         for (uint32_t r = 0; r < nrec; ++r) {
-            const simlab_wall_cell &w = g->walls[r];
+            const simlab_wall_cell &w = g->walls[keep[r]];
             h_mask[r] = w.mask;
             h_cell[r] = w.cell;
             for (int k = 0; k < 9; ++k)
                 h_q[(size_t)k * nrec + r] = (uint32_t)f16_bits_host(w.q[2 * k + 1]) | ((uint32_t)f16_bits_host(w.q[2 * k + 2]) << 16);
             for (int i = 1; i < 19; ++i) put(h_fpost, (size_t)(i - 1) * nrec + r, fe[i]);
-            put(h_mac, r, 1.0f);
-            for (int k = 0; k < 3; ++k) put(h_mac, (size_t)(k + 1) * nrec + r, (float)ui[k]);
-        }
-    }
-    // flags: face type, then solid, then record index + 1 (lattice.rs:489-526)
-    std::vector<uint32_t> h_flags(n, 0);
-    for (uint32_t f : faces) {
-        uint32_t z = f / (nx * ny), r = f - z * nx * ny, y = r / nx, x = r - y * nx;
-        h_flags[f] = (uint32_t)face_type(x, y, z);
-    }
-    if (geom_kind == SIMLAB_GEOM_STATIC) {
-        for (uint64_t k = 0; k < g->nsolid; ++k)
-            if (h_flags[g->solid[k]] == 0) h_flags[g->solid[k]] = 1;
-        for (uint32_t r = 0; r < nrec; ++r) {
-            const uint32_t c = g->walls[r].cell;
-            if ((h_flags[c] & 3u) == 0) h_flags[c] = (r + 1) << 2;
+            put(h_mac, r, 1.0);
+            for (int k = 0; k < 3; ++k) put(h_mac, (size_t)(k + 1) * nrec + r, rb == 8 ? ui[k] : (double)(float)ui[k]);
+            if ((h_flags[w.cell] & 3u) == 0) h_flags[w.cell] = (r + 1) << 2;   // record index + 1
         }
     }
 
@@ -352,8 +385,8 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     hip_check(hipStreamCreateWithFlags(&lat->stream2, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&lat->fork, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&lat->join, hipEventDisableTiming), "hipEventCreate");
-    for (uint64_t c = 0; c < n; ++c) lat->nbulk += h_flags[c] == 0 ? 1 : 0;
-    hip_check(hipHostMalloc((void **)&lat->hist_host, (size_t)KMAX * HIST_STRIDE * 8, hipHostMallocDefault), "hipHostMalloc");
+    for (uint64_t c = 0; c < n; ++c) lat->nbulk += (h_flags[c] == 0 || (h_flags[c] & FLAG_HALFWAY)) ? 1 : 0;
+    hip_check(hipHostMalloc((void **)&lat->hist_host, (size_t)2 * KMAX * HIST_STRIDE * 8, hipHostMallocDefault), "hipHostMalloc");
     hipStream_t st = lat->stream;
     auto up = [&](DevBuf &b, const void *src, size_t bytes) {
         if (bytes) hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), "upload");
@@ -391,7 +424,6 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.nface = lat->nface;
     L.faces = lat->faces.as<uint32_t>();
     L.hist = lat->hist.as<double>();
-    L.counter = lat->counter.as<uint32_t>();
     L.tau0 = 0.5 + 3.0 * su->nu;   // lattice.rs:328
     L.smag2 = su->smagorinsky * su->smagorinsky;
     L.sponge_tau = su->sponge_tau;
@@ -420,8 +452,18 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.slicedef = lat->slicedef.as<int4>();
     L.sacc = lat->sacc.as<float>();
     choose_shape(L, default_block_threads(), 0);
+    {
+        std::vector<uint32_t> hw;
+        for (uint64_t c = 0; c < n; ++c)
+            if (h_flags[c] & FLAG_HALFWAY) hw.push_back((uint32_t)c);
+        L.nhw = (uint32_t)hw.size();
+        L.hw_blocks = (L.nhw + SPECIAL_BLOCK - 1) / SPECIAL_BLOCK;
+        lat->hw_cells.alloc(hw.size() * 4, mem);
+        if (!hw.empty()) hip_check(hipMemcpyAsync(lat->hw_cells.p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st), "upload");
+        L.hw_cells = lat->hw_cells.as<uint32_t>();
+    }
     if (const char *e = std::getenv("SIMLAB_CFD_SERIAL")) lat->concurrent = !(e[0] == '1');
-    lat->partials.alloc((size_t)L.wall_blocks * 8 * 8, mem);
+    lat->partials.alloc((size_t)KMAX * (L.wall_blocks + L.hw_blocks) * 8 * 8, mem);
     L.partials = lat->partials.as<double>();
     lat->memory_bytes = mem;
 
@@ -490,6 +532,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
         }
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
+        hip_check(launch_reduce_hist(lat->L, (uint32_t)chunk, st), "force reduction kernel");
         hip_check(hipMemcpyAsync(lat->hist_host, lat->hist.p, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st), "force copy");
         hip_check(hipStreamSynchronize(st), "GPU wait failed");
         if (lat->timing) {
@@ -573,6 +616,7 @@ int simlab_cfd_lattice_info(const simlab_lattice *lat, simlab_lattice_info *out)
     out->moving = lat->moving ? 1 : 0;
     out->tau0 = lat->L.tau0;
     out->bulk_cells = lat->nbulk;
+    out->halfway_cells = lat->nhalfway;
     return SIMLAB_OK;
 }
 
