@@ -668,6 +668,26 @@ __device__ __forceinline__ void face_writes_wave(const DevLattice &L, bool src, 
                                                  const FaceSrc<typename C::T> &s, uint32_t seed) {
     using T = typename C::T;
     uint64_t bal = __ballot(src);
+    if (__popcll(bal) > 4) {
+        // many sources in this wave (a row along a face, or a run of wall
+        // records there): each writes its own face cells
+        if (src) {
+            const bool lo[3] = {!L.per[0] && x == 1, !L.per[1] && y == 1, !L.per[2] && z == 1};
+            const bool hi[3] = {!L.per[0] && x == L.nx - 2, !L.per[1] && y == L.ny - 2, !L.per[2] && z == L.nz - 2};
+            for (int dz = -1; dz <= 1; ++dz) {
+                if ((dz < 0 && !lo[2]) || (dz > 0 && !hi[2])) continue;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    if ((dy < 0 && !lo[1]) || (dy > 0 && !hi[1])) continue;
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if ((dx < 0 && !lo[0]) || (dx > 0 && !hi[0])) continue;
+                        if (dx == 0 && dy == 0 && dz == 0) continue;
+                        face_cell_write<C, Q>(L, (int)x + dx, (int)y + dy, (int)z + dz, s, seed);
+                    }
+                }
+            }
+        }
+        return;
+    }
     const int lane = (int)__lane_id();
     while (bal) {
         const int l = __builtin_ctzll(bal);
@@ -733,6 +753,83 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_faces(DevLatt
         for (int j = 0; j < 6; ++j) s.p[j] = (T)0;
     }
     face_cell_write<C, 1 - P>(L, (int)x, (int)y, (int)z, s, seed);
+}
+
+// Halfway bounce-back without a record (cells flagged FLAG_HALFWAY): the
+// population a cell sent toward the wall last step is read back from the
+// slot it was written to (load_prev_out for direction i: (n, opp i) after a
+// parity-0 step, (n + c_i, i) after a parity-1 step). The wall neighbour is
+// not a face cell and does not collide, so no other thread touches that slot
+// in this step. With q = 1/2 the Bouzidi rule (shader.rs:419) reduces to
+// f_opp = f_post,i - 6 w (c . u_w), and the momentum exchange is the same
+// (c - u_w) f_post,i + (c + u_w) f_b (shader.rs:423-424).
+// wall velocity (omega x x_w) at the halfway point of link i, and c . u_w
+template <class T> struct HalfwayLink {
+    T cx, cy, cz, xwx, xwy, xwz, uwx, uwy, uwz, cu;
+};
+template <int i, class T>
+__device__ __forceinline__ HalfwayLink<T> halfway_link(const DevLattice &L, const StepArgs &a, const Nbr &nb) {
+    HalfwayLink<T> h;
+    h.cx = (T)CX[i];
+    h.cy = (T)CY[i];
+    h.cz = (T)CZ[i];
+    h.xwx = (T)nb.x - (T)L.center[0] + (T)0.5 * h.cx;
+    h.xwy = (T)nb.y - (T)L.center[1] + (T)0.5 * h.cy;
+    h.xwz = (T)nb.z - (T)L.center[2] + (T)0.5 * h.cz;
+    const T omx = (T)a.omega[0], omy = (T)a.omega[1], omz = (T)a.omega[2];
+    h.uwx = omy * h.xwz - omz * h.xwy;
+    h.uwy = omz * h.xwx - omx * h.xwz;
+    h.uwz = omx * h.xwy - omy * h.xwx;
+    h.cu = h.cx * h.uwx + h.cy * h.uwy + h.cz * h.uwz;
+    return h;
+}
+
+// f_post,i of the previous step, read from where it was stored
+template <class C, int P, int i>
+__device__ __forceinline__ typename C::T halfway_fpi(const DevLattice &L, const Nbr &nb) {
+    if constexpr (P == 0) return dec<C>(popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()]);
+    else return dec<C>(popdir<C>(L, OPP[i])[nb.n]);
+}
+
+template <class C, int P>
+__device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
+                                              typename C::T f[19]) {
+    using T = typename C::T;
+    sfor<18>([&](auto I) {
+        constexpr int i = I + 1;
+        if (m & (1u << i)) {
+            const T wi = wgt<T>(i);
+            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
+            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            f[OPP[i]] = (fpi - (T)6 * wi * h.cu) - wi;
+        }
+    });
+}
+
+// the momentum exchange of those links (the slots read above are unchanged
+// by this step's own stores)
+template <class C, int P>
+__device__ __forceinline__ void halfway_force(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
+                                              typename C::T F[3], typename C::T Tq[3]) {
+    using T = typename C::T;
+    sfor<18>([&](auto I) {
+        constexpr int i = I + 1;
+        if (m & (1u << i)) {
+            const T wi = wgt<T>(i);
+            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
+            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            const T fb = fpi - (T)6 * wi * h.cu;
+            const T flx = (h.cx - h.uwx) * fpi + (h.cx + h.uwx) * fb;
+            const T fly = (h.cy - h.uwy) * fpi + (h.cy + h.uwy) * fb;
+            const T flz = (h.cz - h.uwz) * fpi + (h.cz + h.uwz) * fb;
+            F[0] = F[0] + flx;
+            F[1] = F[1] + fly;
+            F[2] = F[2] + flz;
+            Tq[0] = Tq[0] + (h.xwy * flz - h.xwz * fly);
+            Tq[1] = Tq[1] + (h.xwz * flx - h.xwx * flz);
+            Tq[2] = Tq[2] + (h.xwx * fly - h.xwy * flx);
+        }
+    });
 }
 
 // Block sum of F, T into the block's partial slot of this step; the sums
@@ -927,83 +1024,6 @@ __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
 // (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
 // xblocks pieces when nx > bx) at one z, so the +-1 x-offsets of parity 1
 // stay inside the block and waves touch whole cache lines.
-// Halfway bounce-back without a record (cells flagged FLAG_HALFWAY): the
-// population a cell sent toward the wall last step is read back from the
-// slot it was written to (load_prev_out for direction i: (n, opp i) after a
-// parity-0 step, (n + c_i, i) after a parity-1 step). The wall neighbour is
-// not a face cell and does not collide, so no other thread touches that slot
-// in this step. With q = 1/2 the Bouzidi rule (shader.rs:419) reduces to
-// f_opp = f_post,i - 6 w (c . u_w), and the momentum exchange is the same
-// (c - u_w) f_post,i + (c + u_w) f_b (shader.rs:423-424).
-// wall velocity (omega x x_w) at the halfway point of link i, and c . u_w
-template <class T> struct HalfwayLink {
-    T cx, cy, cz, xwx, xwy, xwz, uwx, uwy, uwz, cu;
-};
-template <int i, class T>
-__device__ __forceinline__ HalfwayLink<T> halfway_link(const DevLattice &L, const StepArgs &a, const Nbr &nb) {
-    HalfwayLink<T> h;
-    h.cx = (T)CX[i];
-    h.cy = (T)CY[i];
-    h.cz = (T)CZ[i];
-    h.xwx = (T)nb.x - (T)L.center[0] + (T)0.5 * h.cx;
-    h.xwy = (T)nb.y - (T)L.center[1] + (T)0.5 * h.cy;
-    h.xwz = (T)nb.z - (T)L.center[2] + (T)0.5 * h.cz;
-    const T omx = (T)a.omega[0], omy = (T)a.omega[1], omz = (T)a.omega[2];
-    h.uwx = omy * h.xwz - omz * h.xwy;
-    h.uwy = omz * h.xwx - omx * h.xwz;
-    h.uwz = omx * h.xwy - omy * h.xwx;
-    h.cu = h.cx * h.uwx + h.cy * h.uwy + h.cz * h.uwz;
-    return h;
-}
-
-// f_post,i of the previous step, read from where it was stored
-template <class C, int P, int i>
-__device__ __forceinline__ typename C::T halfway_fpi(const DevLattice &L, const Nbr &nb) {
-    if constexpr (P == 0) return dec<C>(popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()]);
-    else return dec<C>(popdir<C>(L, OPP[i])[nb.n]);
-}
-
-template <class C, int P>
-__device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
-                                              typename C::T f[19]) {
-    using T = typename C::T;
-    sfor<18>([&](auto I) {
-        constexpr int i = I + 1;
-        if (m & (1u << i)) {
-            const T wi = wgt<T>(i);
-            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
-            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
-            f[OPP[i]] = (fpi - (T)6 * wi * h.cu) - wi;
-        }
-    });
-}
-
-// the momentum exchange of those links (the slots read above are unchanged
-// by this step's own stores)
-template <class C, int P>
-__device__ __forceinline__ void halfway_force(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
-                                              typename C::T F[3], typename C::T Tq[3]) {
-    using T = typename C::T;
-    sfor<18>([&](auto I) {
-        constexpr int i = I + 1;
-        if (m & (1u << i)) {
-            const T wi = wgt<T>(i);
-            const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
-            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
-            const T fb = fpi - (T)6 * wi * h.cu;
-            const T flx = (h.cx - h.uwx) * fpi + (h.cx + h.uwx) * fb;
-            const T fly = (h.cy - h.uwy) * fpi + (h.cy + h.uwy) * fb;
-            const T flz = (h.cz - h.uwz) * fpi + (h.cz + h.uwz) * fb;
-            F[0] = F[0] + flx;
-            F[1] = F[1] + fly;
-            F[2] = F[2] + flz;
-            Tq[0] = Tq[0] + (h.xwy * flz - h.xwz * fly);
-            Tq[1] = Tq[1] + (h.xwz * flx - h.xwx * flz);
-            Tq[2] = Tq[2] + (h.xwx * fly - h.xwy * flx);
-        }
-    });
-}
-
 template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     using T = typename C::T;

@@ -29,7 +29,7 @@ struct HipError : std::runtime_error {
 
 void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) {
-        hipGetLastError();   // clear the sticky error of failed launches
+        (void)hipGetLastError();   // clear the sticky error of failed launches
         throw HipError(e == hipErrorOutOfMemory ? SIMLAB_ERR_OOM : SIMLAB_ERR_GPU,
                        std::string(what) + ": " + hipGetErrorString(e));
     }
@@ -77,7 +77,7 @@ struct DevBuf {
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
     void alloc(size_t n, size_t &total) {
         bytes = std::max<size_t>(n, 16);
@@ -198,7 +198,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     if (const char *m = validate(su, msg)) return fail(SIMLAB_ERR_INVALID, m);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(SIMLAB_ERR_GPU, "no usable GPU: no HIP device found");
     }
     auto lat = std::make_unique<simlab_lattice>();
@@ -565,7 +565,7 @@ int simlab_cfd_gpu_info(char *buf, size_t len) {
         if (!buf || len == 0) return fail(SIMLAB_ERR_INVALID, "null buffer");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-            hipGetLastError();
+            (void)hipGetLastError();
             return fail(SIMLAB_ERR_GPU, "no usable GPU: no HIP device found");
         }
         int dev = 0;
@@ -597,7 +597,7 @@ int simlab_cfd_lattice_new(const simlab_lattice_setup *setup, const simlab_geome
 
 void simlab_cfd_lattice_free(simlab_lattice *lat) {
     if (!lat) return;
-    hipSetDevice(lat->device);
+    (void)hipSetDevice(lat->device);
     delete lat;
 }
 
