@@ -109,6 +109,26 @@ def traffic_from_profile():
         return None
 
 
+def copy_calibration(reps):
+    """A kernel of known traffic: out = x * 1 over 2^28 floats reads and writes
+    exactly 1 GiB each per launch."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    reps = max(1, min(reps, 20))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_calibration": {"read_bytes": 4 * n, "write_bytes": 4 * n, "launches": reps + 3,
+                                           "ms_per_launch": dt * 1e3, "gbs": 8 * n / dt / 1e9}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,11 +138,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference kernel that scales the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profile into bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the lattice is single-device (the multi-GPU slab split is a design "
               f"sketch, not built); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.steps)
+        return
 
     import simlab_amd as sa
     from simlab_amd import workloads as W

@@ -137,9 +137,12 @@ void dither16(uint32_t n, uint32_t seed, uint16_t d[19]) {
         w = xorshift32(w);
     }
 }
+// No clamp before the conversion: round toward zero saturates finite values
+// beyond the half range at +-65504 (as v_cvt_pkrtz_f16_f32 does; the
+// reference clamps at +-65000 instead, shader.rs:516 — a difference only for
+// |f~| > 1.98, far outside a stable flow).
 uint16_t enc_f16_sr(float x, uint16_t d) {
-    float v = std::min(std::max(x * FS, -65000.0f), 65000.0f);
-    return float_to_half_rtz(bitsf(fbits(v) + (uint32_t)(d >> 3)));
+    return float_to_half_rtz(bitsf(fbits(x * FS) + (uint32_t)(d >> 3)));
 }
 uint16_t enc_bf16_sr(float x, uint16_t d) { return (uint16_t)((fbits(x) + (uint32_t)d) >> 16); }
 float dec_bf16(uint16_t b) { return bitsf((uint32_t)b << 16); }

@@ -88,8 +88,9 @@ template <class C> __device__ __forceinline__ typename C::T dec(typename C::E e)
 // d16: 16 dither bits (ignored by the plain storages)
 template <class C> __device__ __forceinline__ typename C::E enc(typename C::T v, uint32_t d16) {
     if constexpr (std::is_same_v<C, CF16>) {
-        float s = __builtin_amdgcn_fmed3f(v * FS, -65000.0f, 65000.0f);
-        float t = __uint_as_float(__float_as_uint(s) + (d16 >> 3));
+        // no clamp: the round-toward-zero conversion saturates finite values
+        // beyond the half range at +-65504
+        float t = __uint_as_float(__float_as_uint(v * FS) + (d16 >> 3));
         half2_t h = __builtin_amdgcn_cvt_pkrtz(t, t);
         return __builtin_bit_cast(uint16_t, h[0]);
     } else if constexpr (std::is_same_v<C, CBF16>) {

@@ -161,6 +161,14 @@ def test_f16_host_rounding_and_conversions():
         assert O.half_to_float(h) == pytest.approx(np.float16(x), rel=0, abs=0)
         assert O.half_to_float(O.f16_rtz(x)) <= x
     assert O.f16_rne(2049.0) == np.float16(2049.0).view(np.uint16)
+    # round toward zero saturates finite overflow at the largest half (the
+    # behaviour of v_cvt_pkrtz_f16_f32), keeps infinities and NaN
+    assert O.f16_rtz(65519.0) == 0x7BFF and O.f16_rtz(1e30) == 0x7BFF and O.f16_rtz(-7e4) == 0xFBFF
+    assert O.f16_rtz(float("inf")) == 0x7C00 and (O.f16_rtz(float("nan")) & 0x7FFF) == 0x7E00
+    assert O.f16_rtz(65503.9) == 0x7BFE and O.f16_rtz(6e-8) == 0x0001
+    # the storage encoder: scaled by 2^15, dithered, saturating
+    assert O.enc_f16_sr(3.0, 0xFFFF) == 0x7BFF and O.enc_f16_sr(-3.0, 0) == 0xFBFF
+    assert O.enc_f16_sr(0.5, 0) == np.float16(16384.0).view(np.uint16)
 
 
 def test_sround_is_unbiased_and_within_one_ulp():
