@@ -233,6 +233,28 @@ int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
 int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
                               double *main_ms, uint64_t *main_launches);
 
+/* Partial bounce-back (new; no counterpart in the reference): solid fraction
+ * phi in [0, 1] per cell (cells x floats, x fastest; NULL sets 0). A cell
+ * blends its post-collision populations with its bounced inputs,
+ * f_i <- (1 - phi) f_i + phi f_in,opp(i), and the porous solid takes the
+ * force phi * sum_i c_i (f_i - f_in,opp(i)). Needs FP32 or FP64 storage and
+ * a lattice without wall geometry, body force, sponge or wall model. */
+int simlab_cfd_lattice_set_solid_fraction(simlab_lattice *lat, const float *phi);
+
+/* Drag of the porous solid and its discrete adjoint (new).
+ * adjoint_forward: restarts from the initial state, runs nsteps steps with
+ * partial bounce-back, keeps the populations every ckpt_every steps, and
+ * returns J = mean over steps [avg_from, nsteps) of the force component
+ * `axis` (0 x, 1 y, 2 z), summed in FP64; ft_out (nsteps x 6, may be NULL)
+ * gets the per-step force (moment entries 0).
+ * adjoint_backward: dJ/dphi for every cell (cells x doubles) by the discrete
+ * adjoint of those steps (the same arithmetic type as the storage),
+ * recomputing each ckpt_every-step segment from its checkpoint. Needs
+ * smagorinsky = 0. */
+int simlab_cfd_adjoint_forward(simlab_lattice *lat, size_t nsteps, size_t ckpt_every, size_t avg_from, int axis,
+                               double *J, double *ft_out);
+int simlab_cfd_adjoint_backward(simlab_lattice *lat, double *dJ_dphi);
+
 /* `voxelize_static` (voxel.rs:348-352, 363-490) on the host CPU: solid cells
  * and wall links of a triangle surface given in lattice coordinates (cell
  * centres at integer points). verts: nverts x 3 doubles; tris: ntris x 3. */

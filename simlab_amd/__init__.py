@@ -119,6 +119,9 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_lattice_set_pops": (I, [V, P, C.c_size_t]),
         "simlab_cfd_lattice_set_timing": (I, [V, I]),
         "simlab_cfd_lattice_timing": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64)]),
+        "simlab_cfd_lattice_set_solid_fraction": (I, [V, P]),
+        "simlab_cfd_adjoint_forward": (I, [V, C.c_size_t, C.c_size_t, C.c_size_t, I, C.POINTER(D), P]),
+        "simlab_cfd_adjoint_backward": (I, [V, P]),
         "simlab_cfd_voxelize_static": (I, [P, U64, P, U64, C.POINTER(U64), C.POINTER(V)]),
         "simlab_cfd_voxels_view": (I, [V, C.POINTER(U64), C.POINTER(P), C.POINTER(U64), C.POINTER(P),
                                        C.POINTER(U64), C.POINTER(U64)]),
@@ -491,6 +494,32 @@ class Lattice:
     def set_pops(self, pops: np.ndarray):
         a = np.ascontiguousarray(pops, dtype=_POP_DTYPE[self.storage])
         _check(self._lib, self._lib.simlab_cfd_lattice_set_pops(self._h, _ptr(a), a.nbytes))
+
+    # ---- partial bounce-back and the drag adjoint (no counterpart in the reference)
+    def set_solid_fraction(self, phi):
+        """Solid fraction per cell (x fastest, values in [0, 1]; None: all 0)."""
+        if phi is None:
+            _check(self._lib, self._lib.simlab_cfd_lattice_set_solid_fraction(self._h, None))
+            return
+        a = np.ascontiguousarray(np.asarray(phi, np.float32).reshape(-1))
+        if a.size != self.setup.cells():
+            raise SimlabError("solid fraction needs one value per cell")
+        _check(self._lib, self._lib.simlab_cfd_lattice_set_solid_fraction(self._h, _ptr(a)))
+
+    def adjoint_forward(self, nsteps: int, ckpt_every: int = 100, avg_from: int = 0, axis: int = 0):
+        """J = mean force component `axis` over steps [avg_from, nsteps) from the
+        initial state, with checkpoints for adjoint_backward; returns (J, ft)."""
+        j = C.c_double()
+        ft = np.zeros((int(nsteps), 6), np.float64)
+        _check(self._lib, self._lib.simlab_cfd_adjoint_forward(self._h, int(nsteps), int(ckpt_every), int(avg_from),
+                                                               int(axis), C.byref(j), _ptr(ft)))
+        return j.value, ft
+
+    def adjoint_backward(self) -> np.ndarray:
+        """dJ/dphi per cell of the last adjoint_forward (float64, x fastest)."""
+        g = np.zeros(self.setup.cells(), np.float64)
+        _check(self._lib, self._lib.simlab_cfd_adjoint_backward(self._h, _ptr(g)))
+        return g
 
     def set_timing(self, enable: bool):
         _check(self._lib, self._lib.simlab_cfd_lattice_set_timing(self._h, int(bool(enable))))

@@ -102,6 +102,13 @@ hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipSt
 hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 // force history of the first nsteps steps of a chunk from the partials
 hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t s);
+// the same over an explicit partials array ([nsteps][nparts][8] -> hist)
+hipError_t launch_reduce_partials(const double *partials, uint32_t nparts, uint32_t nsteps, double *hist, hipStream_t s);
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
+// partial bounce-back: one forward step (FP32/FP64 storage), and the adjoint
+// of one step (A.pops: adjoint populations in the storage's type; grad: d/d phi)
+hipError_t launch_pbb_step(const DevLattice &L, int parity, const StepArgs &a, const float *phi, hipStream_t s);
+hipError_t launch_adjoint_step(const DevLattice &L, const DevLattice &A, int parity, const float *phi, double *grad,
+                               double wj, int axis, hipStream_t s);
 
 }  // namespace simlab

@@ -1057,6 +1057,327 @@ __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- partial bounce-back and its adjoint
+// Partial bounce-back (new; the reference has no porous cells): a cell with
+// solid fraction phi blends its post-collision populations with the bounced
+// inputs, f*_i = (1 - phi) f^c_i + phi f^in_opp(i) (w_i = w_opp(i), so the
+// same holds for the deviations f~), and the porous solid takes the force
+// F = phi sum_i c_i (f^c_i - f^in_opp(i)) (minus the fluid's momentum change).
+// The plain regularized collision (Cs = 0, no body force, no sponge) is the
+// forward map the discrete adjoint differentiates.
+
+// forward intermediates of the regularized collision, for its vector-Jacobian product
+template <class T> struct RegState {
+    T rho1, rho, ux, uy, uz, k1;
+};
+
+template <class T> __device__ __forceinline__ T step_k1(const DevLattice &L) {
+    // 1 - 1/tau with tau as collide() computes it for Cs = 0
+    const T tau0 = (T)L.tau0;
+    const T tau = (T)0.5 * (tau0 + sqrt_(tau0 * tau0));
+    return (T)1 - rcp_(tau);
+}
+
+// adjoint of the moments -> (rho1, j, S) -> f: fbar_k = r1 + c_k.jb + c_k c_k : Sb
+template <class T>
+__device__ __forceinline__ void moments_vjp(T r1, T jbx, T jby, T jbz, const T sb[6], T fb[19]) {
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        T v = r1 + cdot<i>(jbx, jby, jbz);
+        if constexpr (CX[i] != 0) v = v + sb[0];
+        if constexpr (CY[i] != 0) v = v + sb[1];
+        if constexpr (CZ[i] != 0) v = v + sb[2];
+        if constexpr (CX[i] * CY[i] != 0) v = v + (T)(CX[i] * CY[i]) * sb[3];
+        if constexpr (CX[i] * CZ[i] != 0) v = v + (T)(CX[i] * CZ[i]) * sb[4];
+        if constexpr (CY[i] * CZ[i] != 0) v = v + (T)(CY[i] * CZ[i]) * sb[5];
+        fb[i] = v;
+    });
+}
+
+// Given the adjoint of (rho1 through rho, u, Pi) of a state computed from f
+// (rho = 1 + rho1, u = j / rho, Pi = S - rho u u - rho1/3 I), add into fb.
+template <class T>
+__device__ __forceinline__ void macro_vjp(T rho, T ux, T uy, T uz, T rbar, T ubx, T uby, T ubz, const T pb[6], T fb[19]) {
+    // Pi = S - rho u u - rho1/3 I
+    T sb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sb[k] = pb[k];
+    rbar = rbar - (ux * ux * pb[0] + uy * uy * pb[1] + uz * uz * pb[2] + ux * uy * pb[3] + ux * uz * pb[4] + uy * uz * pb[5]);
+    rbar = rbar - (pb[0] + pb[1] + pb[2]) * ((T)1 / (T)3);
+    ubx = ubx - rho * ((T)2 * ux * pb[0] + uy * pb[3] + uz * pb[4]);
+    uby = uby - rho * ((T)2 * uy * pb[1] + ux * pb[3] + uz * pb[5]);
+    ubz = ubz - rho * ((T)2 * uz * pb[2] + ux * pb[4] + uy * pb[5]);
+    // u = j / rho
+    const T ir = rcp_(rho);
+    rbar = rbar - (ubx * ux + uby * uy + ubz * uz) * ir;
+    moments_vjp<T>(rbar, ubx * ir, uby * ir, ubz * ir, sb, fb);
+}
+
+// vector-Jacobian product of the plain regularized collision at the state
+// st: fb = (d out / d f)^T ab
+template <class T> __device__ __forceinline__ void collide_vjp(const RegState<T> &st, const T ab[19], T fb[19]) {
+    const T usq = st.ux * st.ux + st.uy * st.uy + st.uz * st.uz;
+    T rbar = 0, r1bar = 0, ubx = 0, uby = 0, ubz = 0;
+    T pb[6] = {0, 0, 0, 0, 0, 0};
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        const T w = wgt<T>(i);
+        const T a = ab[i] * w;
+        const T cu = cdot<i>(st.ux, st.uy, st.uz);
+        r1bar = r1bar + a;
+        rbar = rbar + a * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq);
+        const T g = st.rho * ((T)3 + (T)9 * cu);
+        if constexpr (CX[i] != 0) ubx = ubx + a * (T)CX[i] * g;
+        if constexpr (CY[i] != 0) uby = uby + a * (T)CY[i] * g;
+        if constexpr (CZ[i] != 0) ubz = ubz + a * (T)CZ[i] * g;
+        ubx = ubx - a * (T)3 * st.rho * st.ux;
+        uby = uby - a * (T)3 * st.rho * st.uy;
+        ubz = ubz - a * (T)3 * st.rho * st.uz;
+        // k1 * 4.5 w (sum_ab c_a c_b P_ab - tr P / 3)
+        const T q = st.k1 * (T)4.5 * a;
+        pb[0] = pb[0] + q * ((T)(CX[i] * CX[i]) - (T)1 / (T)3);
+        pb[1] = pb[1] + q * ((T)(CY[i] * CY[i]) - (T)1 / (T)3);
+        pb[2] = pb[2] + q * ((T)(CZ[i] * CZ[i]) - (T)1 / (T)3);
+        if constexpr (CX[i] * CY[i] != 0) pb[3] = pb[3] + q * (T)(2 * CX[i] * CY[i]);
+        if constexpr (CX[i] * CZ[i] != 0) pb[4] = pb[4] + q * (T)(2 * CX[i] * CZ[i]);
+        if constexpr (CY[i] * CZ[i] != 0) pb[5] = pb[5] + q * (T)(2 * CY[i] * CZ[i]);
+    });
+    macro_vjp<T>(st.rho, st.ux, st.uy, st.uz, rbar + r1bar, ubx, uby, ubz, pb, fb);
+}
+
+template <class T> __device__ __forceinline__ RegState<T> reg_state(const DevLattice &L, const T f[19]) {
+    const Mom<T> m = moments(f);
+    RegState<T> st;
+    st.rho1 = m.rho1;
+    st.rho = (T)1 + m.rho1;
+    const T ir = rcp_(st.rho);
+    st.ux = m.jx * ir;
+    st.uy = m.jy * ir;
+    st.uz = m.jz * ir;
+    st.k1 = step_k1<T>(L);
+    return st;
+}
+
+// one forward step of a lattice with partial bounce-back: the plain
+// collision, the blend, the porous force (per-block FP64 partials of the
+// step's slot), the fused face update. P: parity.
+template <class C, int P>
+__global__ void __launch_bounds__(1024) k_pbb_step(DevLattice L, StepArgs a, const float *phi) {
+    using T = typename C::T;
+    __shared__ double sh[16][6];
+    const uint32_t xb = blockIdx.x % L.xblocks, yb = blockIdx.x / L.xblocks;
+    const uint32_t x = xb * L.bx + threadIdx.x;
+    const uint32_t y = yb * L.by + threadIdx.y;
+    const uint32_t z = blockIdx.y;
+    const bool inside = x < L.nx && y < L.ny;
+    const uint32_t n = (z * L.ny + y) * L.nx + x;
+    const uint32_t fg = inside ? L.flags[n] : 1u;
+    const bool mine = (fg & 3u) == 0;
+    T F[3] = {0, 0, 0};
+    T f[19];
+    FaceSrc<T> s;
+    bool src = false;
+    if (mine) {
+        Nbr nb;
+        nb.make(L, x, y, z, n);
+        gather_pre<C, P>(L, nb, f);
+        T fin[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) fin[i] = f[i];
+        T rho, u[3], pp[6];
+        collide<T, false, false>(L, x, y, z, f, rho, u, NoHook{}, pp);
+        const T ph = (T)phi[n];
+        if (ph != (T)0) {
+            sfor<19>([&](auto I) {
+                constexpr int i = I;
+                const T d = f[i] - fin[OPP[i]];
+                if constexpr (CX[i] != 0) F[0] = F[0] + (T)CX[i] * ph * d;
+                if constexpr (CY[i] != 0) F[1] = F[1] + (T)CY[i] * ph * d;
+                if constexpr (CZ[i] != 0) F[2] = F[2] + (T)CZ[i] * ph * d;
+            });
+#pragma unroll
+            for (int i = 0; i < 19; ++i) f[i] = ((T)1 - ph) * f[i] + ph * fin[OPP[i]];
+        }
+        src = is_face_source(L, x, y, z);
+        if (src) s = face_source(L, f);
+        scatter_post<C, P, false>(L, nb, f, a.seed);
+    }
+    if (__ballot(src)) face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
+    double v[6] = {(double)F[0], (double)F[1], (double)F[2], 0.0, 0.0, 0.0};
+    const uint32_t tid = threadIdx.y * blockDim.x + threadIdx.x;
+    block_sum6(v, sh, tid, blockDim.x * blockDim.y);
+    if (tid == 0) {
+        const uint32_t nblk = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+        double *o = &L.partials[((size_t)a.slot * nblk + bid) * 8];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) o[j] = v[j];
+    }
+}
+
+// Adjoint of the face update that followed a step of parity P (the face
+// cells' inputs of the next step, written from their source cells' outputs).
+// One thread per source cell: it recomputes its own step-P output f* from
+// the step's input state `fwd`, reads the adjoints of the face slots it
+// wrote, adds their vector-Jacobian product to the adjoints of its own
+// output slots, and clears the face slots' adjoints.
+template <class C, int P>
+__global__ void __launch_bounds__(256) k_adj_faces(DevLattice L, DevLattice A, const float *phi) {
+    using T = typename C::T;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= L.ncell) return;
+    uint32_t x, y, z;
+    coords(L, m, x, y, z);
+    if ((L.flags[m] & 3u) != 0 || !is_face_source(L, x, y, z)) return;
+    Nbr nb;
+    nb.make(L, x, y, z, m);
+    T f[19], fin[19];
+    gather_pre<C, P>(L, nb, f);
+#pragma unroll
+    for (int i = 0; i < 19; ++i) fin[i] = f[i];
+    T rho, u[3], pp[6];
+    collide<T, false, false>(L, x, y, z, f, rho, u, NoHook{}, pp);
+    const T ph = (T)phi[m];
+    if (ph != (T)0)
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = ((T)1 - ph) * f[i] + ph * fin[OPP[i]];
+    // the source state the face rule saw: rho, u, Pi of f*
+    const FaceSrc<T> s = face_source(L, f);
+    T rbar = 0, ubx = 0, uby = 0, ubz = 0;
+    T pb[6] = {0, 0, 0, 0, 0, 0};
+    T *adj = (T *)A.pops;
+    const bool lo[3] = {!L.per[0] && x == 1, !L.per[1] && y == 1, !L.per[2] && z == 1};
+    const bool hi[3] = {!L.per[0] && x == L.nx - 2, !L.per[1] && y == L.ny - 2, !L.per[2] && z == L.nz - 2};
+    for (int dz = -1; dz <= 1; ++dz) {
+        if ((dz < 0 && !lo[2]) || (dz > 0 && !hi[2])) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            if ((dy < 0 && !lo[1]) || (dy > 0 && !hi[1])) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                if ((dx < 0 && !lo[0]) || (dx > 0 && !hi[0])) continue;
+                if (dx == 0 && dy == 0 && dz == 0) continue;
+                const int bx = (int)x + dx, by = (int)y + dy, bz = (int)z + dz;
+                const uint32_t ty = L.flags[((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx] & 3u;
+                // the face cell's state: velocity face takes rho from inside
+                // and u = u_inf; open face takes rho = 1 and u from inside
+                const T rb = ty == 2u ? s.rho : (T)1;
+                const T vx = ty == 2u ? (T)L.u_inf[0] : s.ux, vy = ty == 2u ? (T)L.u_inf[1] : s.uy,
+                        vz = ty == 2u ? (T)L.u_inf[2] : s.uz;
+                const T usq = vx * vx + vy * vy + vz * vz;
+                T rbb = 0, vbx = 0, vby = 0, vbz = 0;
+                for (int i = 0; i < 19; ++i) {
+                    if (!face_live(L, bx, by, bz, i)) continue;
+                    // the slot face_store<C, P> wrote direction i of b to
+                    const int cx = rcx(i), cy = rcy(i), cz = rcz(i);
+                    uint32_t cell, dir;
+                    if (P == 0 || i == 0) {
+                        cell = ((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx;
+                        dir = (uint32_t)ropp(i);
+                    } else {
+                        int tx = bx + cx, tyy = by + cy, tz = bz + cz;
+                        tx = tx < 0 ? tx + (int)L.nx : (tx >= (int)L.nx ? tx - (int)L.nx : tx);
+                        tyy = tyy < 0 ? tyy + (int)L.ny : (tyy >= (int)L.ny ? tyy - (int)L.ny : tyy);
+                        tz = tz < 0 ? tz + (int)L.nz : (tz >= (int)L.nz ? tz - (int)L.nz : tz);
+                        cell = ((uint32_t)tz * L.ny + (uint32_t)tyy) * L.nx + (uint32_t)tx;
+                        dir = (uint32_t)i;
+                    }
+                    T *slot = adj + (size_t)dir * L.ncell + cell;
+                    const T ab = *slot;
+                    *slot = (T)0;
+                    if (ab == (T)0) continue;
+                    const T w = rwgt<T>(i);
+                    const T cu = (T)cx * vx + (T)cy * vy + (T)cz * vz;
+                    // f_b,i = w (rb - 1 + rb (3 cu + 4.5 cu^2 - 1.5 u^2)) + 4.5 w (c c : Pi - tr Pi / 3)
+                    rbb = rbb + ab * w * ((T)1 + (T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq);
+                    const T g = rb * w * ab * ((T)3 + (T)9 * cu);
+                    vbx = vbx + g * (T)cx - (T)3 * rb * w * ab * vx;
+                    vby = vby + g * (T)cy - (T)3 * rb * w * ab * vy;
+                    vbz = vbz + g * (T)cz - (T)3 * rb * w * ab * vz;
+                    const T q = (T)4.5 * w * ab;
+                    pb[0] = pb[0] + q * ((T)(cx * cx) - (T)1 / (T)3);
+                    pb[1] = pb[1] + q * ((T)(cy * cy) - (T)1 / (T)3);
+                    pb[2] = pb[2] + q * ((T)(cz * cz) - (T)1 / (T)3);
+                    pb[3] = pb[3] + q * (T)(2 * cx * cy);
+                    pb[4] = pb[4] + q * (T)(2 * cx * cz);
+                    pb[5] = pb[5] + q * (T)(2 * cy * cz);
+                }
+                if (ty == 2u) {
+                    rbar = rbar + rbb;
+                } else {
+                    ubx = ubx + vbx;
+                    uby = uby + vby;
+                    ubz = ubz + vbz;
+                }
+            }
+        }
+    }
+    // back through rho, u, Pi of f* to f* itself (u = j / rho, no force)
+    T fb[19];
+    macro_vjp<T>(s.rho, s.ux, s.uy, s.uz, rbar, ubx, uby, ubz, pb, fb);
+    // add to the adjoints of this cell's step-P output slots
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        T *slot;
+        if constexpr (P == 0 || i == 0) slot = adj + (size_t)OPP[i] * L.ncell + nb.n;
+        else slot = adj + (size_t)i * L.ncell + nb.template at<CX[i], CY[i], CZ[i]>();
+        *slot = *slot + fb[i];
+    });
+}
+
+// Adjoint of one step of parity P: each fluid cell reads the adjoints of
+// the slots it wrote, goes back through the blend and the collision at its
+// recomputed forward state (inputs from `fwd` = L.pops), adds the drag
+// functional's derivative (weight wj on this step's x-force), accumulates
+// d/d phi and writes the adjoints of its inputs into the slots it read (the
+// same set of slots: in-place like the forward).
+template <class C, int P>
+__global__ void __launch_bounds__(1024) k_adj_step(DevLattice L, DevLattice A, const float *phi, double *grad, double wj, int axis) {
+    using T = typename C::T;
+    const uint32_t xb = blockIdx.x % L.xblocks, yb = blockIdx.x / L.xblocks;
+    const uint32_t x = xb * L.bx + threadIdx.x;
+    const uint32_t y = yb * L.by + threadIdx.y;
+    const uint32_t z = blockIdx.y;
+    if (x >= L.nx || y >= L.ny) return;
+    const uint32_t n = (z * L.ny + y) * L.nx + x;
+    if ((L.flags[n] & 3u) != 0) return;
+    Nbr nb;
+    nb.make(L, x, y, z, n);
+    T fin[19], fc[19];
+    gather_pre<C, P>(L, nb, fin);
+#pragma unroll
+    for (int i = 0; i < 19; ++i) fc[i] = fin[i];
+    T rho, u[3], pp[6];
+    collide<T, false, false>(L, x, y, z, fc, rho, u, NoHook{}, pp);
+    const RegState<T> st = reg_state<T>(L, fin);
+    const T ph = (T)phi[n];
+    const T w = (T)wj;
+    // adjoints of this cell's outputs (the slots scatter_post<P> wrote)
+    T ao[19];
+    gather_post<C, P>(A, nb, ao);
+    T ac[19], ain[19];
+    T gphi = 0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) ain[i] = 0;
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        const T cax = axis == 0 ? (T)CX[i] : (axis == 1 ? (T)CY[i] : (T)CZ[i]);
+        // blend: f*_i = (1 - phi) fc_i + phi fin_opp(i); force term phi c (fc_i - fin_opp(i))
+        const T d = fc[i] - fin[OPP[i]];
+        gphi = gphi + ao[i] * (fin[OPP[i]] - fc[i]) + w * cax * d;
+        ac[i] = ((T)1 - ph) * ao[i] + w * ph * cax;
+        ain[OPP[i]] = ain[OPP[i]] + ph * ao[i] - w * ph * cax;
+    });
+    T fb[19];
+    collide_vjp<T>(st, ac, fb);
+    grad[n] = grad[n] + (double)gphi;
+    // write the adjoints of the inputs into the slots gather_pre<P> read
+    T *adj = (T *)A.pops;
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        const T v = fb[i] + ain[i];
+        if constexpr (P == 0 || i == 0) adj[(size_t)i * L.ncell + nb.n] = v;
+        else adj[(size_t)OPP[i] * L.ncell + nb.template at<-CX[i], -CY[i], -CZ[i]>()] = v;
+    });
+}
+
 // ---------------------------------------------------------------- init
 template <class C> __global__ void __launch_bounds__(256) k_init(DevLattice L) {
     using T = typename C::T;
@@ -1299,6 +1620,11 @@ hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t 
     return hipGetLastError();
 }
 
+hipError_t launch_reduce_partials(const double *partials, uint32_t nparts, uint32_t nsteps, double *hist, hipStream_t s) {
+    k_reduce_hist<<<nsteps, 256, 0, s>>>(partials, nparts, hist);
+    return hipGetLastError();
+}
+
 hipError_t launch_samplers(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
     // samplers use the module of the other parity (lattice.rs:733-737)
     const int q = 1 - p;
@@ -1320,6 +1646,45 @@ hipError_t launch_samplers(const DevLattice &L, int p, const StepArgs &a, hipStr
         default: SAMP(CF64); break;
     }
 #undef SAMP
+    return hipGetLastError();
+}
+
+hipError_t launch_pbb_step(const DevLattice &L, int p, const StepArgs &a, const float *phi, hipStream_t s) {
+    const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
+    switch (L.storage) {
+        case SIMLAB_STORAGE_FP32:
+            if (p == 0) k_pbb_step<CF32, 0><<<grid, block, 0, s>>>(L, a, phi);
+            else k_pbb_step<CF32, 1><<<grid, block, 0, s>>>(L, a, phi);
+            break;
+        case SIMLAB_STORAGE_FP64:
+            if (p == 0) k_pbb_step<CF64, 0><<<grid, block, 0, s>>>(L, a, phi);
+            else k_pbb_step<CF64, 1><<<grid, block, 0, s>>>(L, a, phi);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(const DevLattice &L, const DevLattice &A, int p, const float *phi, double *grad, double wj,
+                               int axis, hipStream_t s) {
+    const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
+    const uint32_t nb = cdiv(L.ncell, 256);
+#define ADJ(C)                                                                                  \
+    do {                                                                                        \
+        if (p == 0) {                                                                           \
+            k_adj_faces<C, 0><<<nb, 256, 0, s>>>(L, A, phi);                                    \
+            k_adj_step<C, 0><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);                 \
+        } else {                                                                                \
+            k_adj_faces<C, 1><<<nb, 256, 0, s>>>(L, A, phi);                                    \
+            k_adj_step<C, 1><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);                 \
+        }                                                                                       \
+    } while (0)
+    switch (L.storage) {
+        case SIMLAB_STORAGE_FP32: ADJ(CF32); break;
+        case SIMLAB_STORAGE_FP64: ADJ(CF64); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef ADJ
     return hipGetLastError();
 }
 

@@ -85,6 +85,11 @@ struct DevBuf {
         total += bytes;
     }
     template <class T> T *as() const { return (T *)p; }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
 };
 
 }  // namespace
@@ -113,6 +118,13 @@ struct simlab_lattice {
     DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
         counter, dump, mass_part, mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
     bool fused_faces = true;   // face update fused into the previous step
+    // partial bounce-back and the drag adjoint
+    DevBuf phi, grad, adj, ckpt, seg, pbb_part;
+    bool pbb = false;               // solid fraction set
+    bool has_solid = false;         // the geometry marks solid cells
+    uint64_t adj_steps = 0, adj_every = 0, adj_from = 0;
+    int adj_axis = 0;
+    bool adj_ready = false;         // checkpoints of the last adjoint_forward are valid
     bool concurrent = true;    // k_special on a second stream beside k_bulk
     double *hist_host = nullptr;   // pinned, KMAX x HIST_STRIDE
     bool timing = false;
@@ -256,6 +268,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     if (geom_kind == SIMLAB_GEOM_STATIC)
         for (uint64_t k = 0; k < g->nsolid; ++k)
             if (h_flags[g->solid[k]] == 0) h_flags[g->solid[k]] = 1;
+    lat->has_solid = geom_kind == SIMLAB_GEOM_STATIC && g->nsolid > 0;
 
     // Wall cells whose cut links all sit halfway (q = 1/2) and lead into
     // cells that are not face cells need no record: the bulk kernel bounces
@@ -819,6 +832,182 @@ int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms, double
     if (main_ms) *main_ms = lat->last_main_ms;
     if (main_launches) *main_launches = lat->last_main_launches;
     return SIMLAB_OK;
+}
+
+// ---------------------------------------------------------------- partial bounce-back + adjoint
+namespace {
+
+const char *pbb_unsupported(const simlab_lattice *lat) {
+    const simlab_lattice_setup &s = lat->setup;
+    if (lat->storage != SIMLAB_STORAGE_FP32 && lat->storage != SIMLAB_STORAGE_FP64)
+        return "partial bounce-back needs FP32 or FP64 storage";
+    if (lat->nrec != 0 || lat->nhalfway != 0 || lat->has_solid)
+        return "partial bounce-back needs a lattice without wall geometry";
+    if (s.body_force[0] != 0.0 || s.body_force[1] != 0.0 || s.body_force[2] != 0.0)
+        return "partial bounce-back does not take a body force";
+    for (int k = 0; k < 6; ++k)
+        if (s.sponge[k]) return "partial bounce-back does not take a sponge";
+    if (s.wall_model || s.total_pressure_inlet) return "partial bounce-back needs no wall model and no total-pressure inlet";
+    return nullptr;
+}
+
+}  // namespace
+
+int simlab_cfd_lattice_set_solid_fraction(simlab_lattice *lat, const float *phi) {
+    return guarded([&]() -> int {
+        if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+        if (const char *m = pbb_unsupported(lat)) return fail(SIMLAB_ERR_INVALID, m);
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            uint64_t mem = 0;
+            if (!lat->phi.p) lat->phi.alloc(lat->ncell * 4, mem);
+            lat->memory_bytes += mem;
+            if (phi) {
+                for (uint64_t c = 0; c < lat->ncell; ++c)
+                    if (!(phi[c] >= 0.0f && phi[c] <= 1.0f)) return fail(SIMLAB_ERR_INVALID, "solid fraction outside [0, 1]");
+                hip_check(hipMemcpy(lat->phi.p, phi, lat->ncell * 4, hipMemcpyHostToDevice), "upload");
+            } else {
+                hip_check(hipMemset(lat->phi.p, 0, lat->ncell * 4), "memset");
+            }
+            lat->pbb = true;
+            lat->adj_ready = false;
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_adjoint_forward(simlab_lattice *lat, size_t nsteps, size_t ckpt_every, size_t avg_from, int axis,
+                               double *J, double *ft_out) {
+    return guarded([&]() -> int {
+        if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+        if (!lat->pbb) return fail(SIMLAB_ERR_STATE, "set a solid fraction first");
+        if (nsteps == 0 || ckpt_every == 0 || avg_from >= nsteps || axis < 0 || axis > 2)
+            return fail(SIMLAB_ERR_INVALID, "need nsteps > avg_from >= 0, ckpt_every > 0 and axis 0, 1 or 2");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            DevLattice &L = lat->L;
+            hipStream_t st = lat->stream;
+            const size_t state = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            const size_t nck = (nsteps + ckpt_every - 1) / ckpt_every;
+            const uint32_t nblk = L.xblocks * L.yblocks * L.nz;
+            uint64_t mem = 0;
+            try {
+                if (lat->ckpt.bytes < nck * state) { lat->ckpt.reset(); lat->ckpt.alloc(nck * state, mem); }
+                if (lat->pbb_part.bytes < (size_t)KMAX * nblk * 64) { lat->pbb_part.reset(); lat->pbb_part.alloc((size_t)KMAX * nblk * 64, mem); }
+            } catch (const HipError &e) {
+                if (e.code == SIMLAB_ERR_OOM) return fail(SIMLAB_ERR_OOM, "not enough GPU memory for the adjoint checkpoints");
+                throw;
+            }
+            lat->memory_bytes += mem;
+            // from the initial state, as a fresh lattice
+            hip_check(launch_init(L, st), "init kernel");
+            lat->steps_done = 0;
+            StepArgs a0;
+            simlab_step_in idn{};
+            for (int k = 0; k < 3; ++k) idn.rot[k][k] = idn.rot_prev[k][k] = 1.0;
+            pack_step(idn, 0, 0, a0);
+            hip_check(launch_faces(L, 0, a0.seed, st), "face kernel");
+            hip_check(launch_faces(L, 1, 2654435761u, st), "face kernel");
+            double sum = 0.0;
+            size_t done = 0;
+            while (done < nsteps) {
+                const size_t chunk = std::min<size_t>(KMAX, nsteps - done);
+                for (size_t k = 0; k < chunk; ++k) {
+                    const uint64_t t = done + k;
+                    if (t % ckpt_every == 0)
+                        hip_check(hipMemcpyAsync((char *)lat->ckpt.p + (t / ckpt_every) * state, L.pops, state,
+                                                 hipMemcpyDeviceToDevice, st), "checkpoint copy");
+                    StepArgs a;
+                    pack_step(idn, t, (uint32_t)k, a);
+                    DevLattice Lp = L;
+                    Lp.partials = lat->pbb_part.as<double>();
+                    hip_check(launch_pbb_step(Lp, (int)(t % 2), a, lat->phi.as<float>(), st), "partial bounce-back step");
+                }
+                hip_check(launch_reduce_partials(lat->pbb_part.as<double>(), nblk, (uint32_t)chunk, L.hist, st), "force reduction kernel");
+                hip_check(hipMemcpyAsync(lat->hist_host, L.hist, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st), "force copy");
+                hip_check(hipStreamSynchronize(st), "GPU wait failed");
+                for (size_t k = 0; k < chunk; ++k) {
+                    const uint64_t t = done + k;
+                    if (t >= avg_from) sum += lat->hist_host[k * HIST_STRIDE + axis];
+                    if (ft_out)
+                        for (int j = 0; j < 6; ++j) ft_out[t * 6 + j] = lat->hist_host[k * HIST_STRIDE + j];
+                }
+                done += chunk;
+            }
+            lat->steps_done = nsteps;
+            lat->adj_steps = nsteps;
+            lat->adj_every = ckpt_every;
+            lat->adj_from = avg_from;
+            lat->adj_axis = axis;
+            lat->adj_ready = true;
+            if (J) *J = sum / (double)(nsteps - avg_from);
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_adjoint_backward(simlab_lattice *lat, double *dJ_dphi) {
+    return guarded([&]() -> int {
+        if (!lat || !dJ_dphi) return fail(SIMLAB_ERR_INVALID, "null argument");
+        if (!lat->adj_ready) return fail(SIMLAB_ERR_STATE, "run adjoint_forward first");
+        if (lat->setup.smagorinsky != 0.0) return fail(SIMLAB_ERR_INVALID, "the adjoint needs smagorinsky = 0");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            DevLattice &L = lat->L;
+            hipStream_t st = lat->stream;
+            const size_t state = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            const uint64_t T = lat->adj_steps, K = lat->adj_every;
+            uint64_t mem = 0;
+            try {
+                if (lat->seg.bytes < (size_t)K * state) { lat->seg.reset(); lat->seg.alloc((size_t)K * state, mem); }
+                if (!lat->adj.p) lat->adj.alloc(state, mem);
+                if (!lat->grad.p) lat->grad.alloc(lat->ncell * 8, mem);
+            } catch (const HipError &e) {
+                if (e.code == SIMLAB_ERR_OOM) return fail(SIMLAB_ERR_OOM, "not enough GPU memory for the adjoint segment");
+                throw;
+            }
+            lat->memory_bytes += mem;
+            hip_check(hipMemsetAsync(lat->adj.p, 0, state, st), "memset");
+            hip_check(hipMemsetAsync(lat->grad.p, 0, lat->ncell * 8, st), "memset");
+            DevLattice A = L;
+            A.pops = lat->adj.p;
+            const double wj = 1.0 / (double)(T - lat->adj_from);
+            simlab_step_in idn{};
+            for (int k = 0; k < 3; ++k) idn.rot[k][k] = idn.rot_prev[k][k] = 1.0;
+            const uint64_t nseg = (T + K - 1) / K;
+            for (uint64_t sg = nseg; sg-- > 0;) {
+                const uint64_t t0 = sg * K, t1 = std::min<uint64_t>(T, t0 + K);
+                // recompute the segment: seg[k] = input state of step t0 + k
+                char *S = (char *)lat->seg.p;
+                hip_check(hipMemcpyAsync(S, (char *)lat->ckpt.p + sg * state, state, hipMemcpyDeviceToDevice, st), "copy");
+                for (uint64_t t = t0; t + 1 < t1; ++t) {
+                    char *next = S + (t + 1 - t0) * state;
+                    hip_check(hipMemcpyAsync(next, S + (t - t0) * state, state, hipMemcpyDeviceToDevice, st), "copy");
+                    DevLattice Ls = L;
+                    Ls.pops = next;
+                    Ls.partials = lat->pbb_part.as<double>();
+                    StepArgs a;
+                    pack_step(idn, t, 0, a);
+                    hip_check(launch_pbb_step(Ls, (int)(t % 2), a, lat->phi.as<float>(), st), "partial bounce-back step");
+                }
+                for (uint64_t t = t1; t-- > t0;) {
+                    DevLattice Ls = L;
+                    Ls.pops = S + (t - t0) * state;
+                    hip_check(launch_adjoint_step(Ls, A, (int)(t % 2), lat->phi.as<float>(), lat->grad.as<double>(),
+                                                  t >= lat->adj_from ? wj : 0.0, lat->adj_axis, st), "adjoint step");
+                }
+            }
+            hip_check(hipMemcpyAsync(dJ_dphi, lat->grad.p, lat->ncell * 8, hipMemcpyDeviceToHost, st), "GPU read failed");
+            hip_check(hipStreamSynchronize(st), "GPU wait failed");
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
 }
 
 }  // extern "C"

@@ -100,3 +100,23 @@ def sphere_channel_small(storage: Storage = Storage.FP32, debug_dump=False):
 def sphere_channel_large(storage: Storage = Storage.FP32, debug_dump=False):
     """256x128x128, sphere r=16 at x=64, u=0.05, Re=1000, Smagorinsky Cs=0.1 (5000 steps in full)."""
     return sphere_channel((256, 128, 128), 16.0, 64.0, 1000.0, 0.05, 0.1, storage, debug_dump)
+
+
+def porous_box(dims=(128, 64, 64), box=24, x0=40.0, u=0.04, re=200.0, seed=20261203,
+               storage: Storage = Storage.FP32, debug_dump=False):
+    """Drag-sensitivity case: a seeded random solid fraction in [0, 1] inside a
+    box x box x box cube centred at (x0, ny/2, nz/2), zero elsewhere, under
+    partial bounce-back; velocity inlet u at -x, open outlet at +x, periodic
+    in y and z; nu from Re on the box size. Returns (setup, phi, meta); phi
+    is cells float32, x fastest."""
+    import numpy as np
+    nx, ny, nz = dims
+    nu = u * box / re
+    setup = LatticeSetup(dims=dims, faces=(FaceBc.VELOCITY, FaceBc.OPEN, FaceBc.PERIODIC, FaceBc.PERIODIC,
+                                           FaceBc.PERIODIC, FaceBc.PERIODIC),
+                         nu=nu, u_inf=(u, 0.0, 0.0), u_init=(u, 0.0, 0.0), storage=storage, debug_dump=debug_dump)
+    rng = np.random.default_rng(seed)
+    phi = np.zeros((nz, ny, nx), np.float32)
+    lo = [int(round(x0 - box / 2)), ny // 2 - box // 2, nz // 2 - box // 2]
+    phi[lo[2]:lo[2] + box, lo[1]:lo[1] + box, lo[0]:lo[0] + box] = rng.random((box, box, box), dtype=np.float32)
+    return setup, phi.reshape(-1), dict(nu=nu, tau0=0.5 + 3.0 * nu, box_lo=tuple(lo), box=box, u=u, re=re, seed=seed)
