@@ -161,6 +161,7 @@ typedef struct simlab_lattice_info {
 } simlab_lattice_info;
 
 typedef struct simlab_lattice simlab_lattice;            /* opaque */
+typedef struct simlab_batch simlab_batch;                /* opaque */
 typedef struct simlab_static_voxels simlab_static_voxels; /* opaque */
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -232,6 +233,29 @@ int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t byte
 int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
 int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
                               double *main_ms, uint64_t *main_launches);
+
+/* Launch-shape autotuner (new; no counterpart in the reference: wgpu fixes
+ * the workgroup size). Times `steps` steps (default 20) of the lattice for
+ * each candidate tile of the stream-collide kernel (bx x-cells, a multiple
+ * of 64, by rows, 64-1024 threads), keeps the fastest, and restores the
+ * lattice's state (populations, records, steps_done) after every trial.
+ * With use_cache the tile found earlier in this process for a lattice of the
+ * same size, storage and wall presence is taken without timing. tile_out
+ * (may be NULL) gets {bx, by}; step_us_out the best step time (0 if cached). */
+int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cache, uint32_t *tile_out,
+                                double *step_us_out);
+/* Sets the tile of the stream-collide kernel explicitly. */
+int simlab_cfd_lattice_set_tile(simlab_lattice *lat, uint32_t bx, uint32_t by);
+
+/* Batches (new): several independent lattices (one storage, the same body
+ * force and sponge settings, static geometry, the same steps_done) stepped
+ * together, one launch per kernel and step for the whole batch; each lattice
+ * keeps its own tile. The lattices stay owned by the caller and must outlive
+ * the batch. run_still: nsteps still steps of every lattice; ft_out (may be
+ * NULL) is lattices x nsteps x 6 doubles, as `run` gives per lattice. */
+int simlab_cfd_batch_new(simlab_lattice *const *lats, size_t n, simlab_batch **out);
+int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out);
+void simlab_cfd_batch_free(simlab_batch *b);
 
 /* Partial bounce-back (new; no counterpart in the reference): solid fraction
  * phi in [0, 1] per cell (cells x floats, x fastest; NULL sets 0). A cell

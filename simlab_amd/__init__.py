@@ -23,7 +23,7 @@ import numpy as np
 __all__ = [
     "SimlabError", "Storage", "FaceBc", "LatticeSetup", "StepIn", "IDENTITY", "Geometry", "Probes",
     "SliceDef", "Lattice", "StaticVoxels", "voxelize_static", "inside_grid", "gpu_info", "max_cells",
-    "load_library", "LIB_PATH",
+    "load_library", "LIB_PATH", "Batch",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsimlab_cfd.so")
@@ -119,6 +119,11 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_lattice_set_pops": (I, [V, P, C.c_size_t]),
         "simlab_cfd_lattice_set_timing": (I, [V, I]),
         "simlab_cfd_lattice_timing": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64)]),
+        "simlab_cfd_lattice_autotune": (I, [V, C.c_uint32, I, P, C.POINTER(D)]),
+        "simlab_cfd_lattice_set_tile": (I, [V, C.c_uint32, C.c_uint32]),
+        "simlab_cfd_batch_new": (I, [P, C.c_size_t, C.POINTER(V)]),
+        "simlab_cfd_batch_run_still": (I, [V, C.c_size_t, P]),
+        "simlab_cfd_batch_free": (None, [V]),
         "simlab_cfd_lattice_set_solid_fraction": (I, [V, P]),
         "simlab_cfd_adjoint_forward": (I, [V, C.c_size_t, C.c_size_t, C.c_size_t, I, C.POINTER(D), P]),
         "simlab_cfd_adjoint_backward": (I, [V, P]),
@@ -495,6 +500,18 @@ class Lattice:
         a = np.ascontiguousarray(pops, dtype=_POP_DTYPE[self.storage])
         _check(self._lib, self._lib.simlab_cfd_lattice_set_pops(self._h, _ptr(a), a.nbytes))
 
+    # ---- launch-shape autotuner (no counterpart in the reference)
+    def autotune(self, steps: int = 20, use_cache: bool = True):
+        """Times each candidate tile of the stream-collide kernel, keeps the
+        fastest and restores the state; returns ((bx, by), best step us)."""
+        tile = (C.c_uint32 * 2)()
+        us = C.c_double()
+        _check(self._lib, self._lib.simlab_cfd_lattice_autotune(self._h, int(steps), int(bool(use_cache)), tile, C.byref(us)))
+        return (int(tile[0]), int(tile[1])), us.value
+
+    def set_tile(self, bx: int, by: int):
+        _check(self._lib, self._lib.simlab_cfd_lattice_set_tile(self._h, int(bx), int(by)))
+
     # ---- partial bounce-back and the drag adjoint (no counterpart in the reference)
     def set_solid_fraction(self, phi):
         """Solid fraction per cell (x fastest, values in [0, 1]; None: all 0)."""
@@ -529,3 +546,28 @@ class Lattice:
         a, b, c = C.c_double(), C.c_double(), C.c_uint64()
         _check(self._lib, self._lib.simlab_cfd_lattice_timing(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, int(c.value)
+
+
+class Batch:
+    """Several lattices stepped together, one launch per kernel and step
+    (simlab_cfd_batch_*). The lattices must outlive the batch."""
+
+    def __init__(self, lattices):
+        self._lib = load_library()
+        self.lattices = list(lattices)
+        arr = (C.c_void_p * len(self.lattices))(*[l._h for l in self.lattices])
+        h = C.c_void_p()
+        _check(self._lib, self._lib.simlab_cfd_batch_new(arr, len(self.lattices), C.byref(h)))
+        self._h = h
+
+    def run_still(self, nsteps: int) -> np.ndarray:
+        """(lattices, nsteps, 6) forces and moments per step."""
+        out = np.zeros((len(self.lattices), int(nsteps), 6), np.float64)
+        _check(self._lib, self._lib.simlab_cfd_batch_run_still(self._h, int(nsteps), _ptr(out)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.simlab_cfd_batch_free(h)
+            self._h = None

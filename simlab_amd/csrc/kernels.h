@@ -91,6 +91,19 @@ struct StepArgs {
     uint32_t slot;
 };
 
+// A batch of lattices stepped together (device arrays prepared by the host):
+// lats[nl] device views, boff_bulk[nl + 1] / boff_sp[nl + 1] first block of
+// each lattice in the bulk / special launch, `threads` per bulk block (each
+// lattice's bx * by fits in it).
+struct BatchLaunch {
+    const DevLattice *lats;
+    const uint32_t *boff_bulk, *boff_sp;
+    uint32_t nl, nbulk, nsp, threads;
+    int32_t storage;
+    bool force, sponge;
+};
+hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a, hipStream_t s);
+
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
 // standalone face update before a step of this parity (seed of that step)
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);

@@ -837,10 +837,10 @@ __device__ __forceinline__ void halfway_force(const DevLattice &L, const StepArg
 // over blocks are formed after the chunk of steps by k_reduce_hist (in block
 // order: deterministic), so no fence or atomic is needed here.
 __device__ __forceinline__ void reduce_ft(const DevLattice &L, const StepArgs &a, double v[6], double (*sh)[6],
-                                          uint32_t tid, uint32_t bs, uint32_t nblocks) {
+                                          uint32_t tid, uint32_t bs, uint32_t nblocks, uint32_t bid) {
     block_sum6(v, sh, tid, bs);
     if (tid == 0) {
-        double *o = &L.partials[((size_t)a.slot * nblocks + blockIdx.x) * 8];
+        double *o = &L.partials[((size_t)a.slot * nblocks + bid) * 8];
 #pragma unroll
         for (int j = 0; j < 6; ++j) o[j] = v[j];
     }
@@ -873,18 +873,18 @@ __global__ void __launch_bounds__(256) k_reduce_hist(const double *partials, uin
 // post-collision state, which is what the next step's face_gather would
 // read: the face-cell slots it writes are touched by no other thread of
 // this step.
+// block `bid` of a lattice's special grid (SPECIAL_BLOCK threads)
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
+__device__ __forceinline__ void special_block(const DevLattice &L, const StepArgs &a, uint32_t bid, double (*sh)[6]) {
     using T = typename C::T;
-    __shared__ double sh[16][6];
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t tid = threadIdx.x, bs = SPECIAL_BLOCK;
     const uint32_t seed_next = a.seed + 2654435761u;   // lattice.rs:676 seed of the next step
     const uint32_t nblocks = L.wall_blocks + L.hw_blocks;
-    if (blockIdx.x >= L.wall_blocks) {
+    if (bid >= L.wall_blocks) {
         // momentum exchange of the halfway wall cells (their bounce-back is
         // done by k_bulk): reads only last step's outputs, which no thread
         // of this step writes
-        const uint32_t j = (blockIdx.x - L.wall_blocks) * bs + tid;
+        const uint32_t j = (bid - L.wall_blocks) * bs + tid;
         T F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
         if (j < L.nhw) {
             const uint32_t n = L.hw_cells[j];
@@ -896,10 +896,10 @@ __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
             halfway_force<C, P>(L, a, nb, (fg >> 1) & 0x7FFFEu, F, Tq);
         }
         double v[6] = {(double)F[0], (double)F[1], (double)F[2], (double)Tq[0], (double)Tq[1], (double)Tq[2]};
-        reduce_ft(L, a, v, sh, tid, bs, nblocks);
+        reduce_ft(L, a, v, sh, tid, bs, nblocks, bid);
         return;
     }
-    const uint32_t wb = blockIdx.x;
+    const uint32_t wb = bid;
     const uint32_t r = wb * bs + tid;
     T F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
     T pp[6];
@@ -1016,7 +1016,13 @@ __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
     }
     if (__ballot(src)) face_writes_wave<C, P>(L, src, sx, sy, sz, fsrc, seed_next);
     double v[6] = {(double)F[0], (double)F[1], (double)F[2], (double)Tq[0], (double)Tq[1], (double)Tq[2]};
-    reduce_ft(L, a, v, sh, tid, bs, nblocks);
+    reduce_ft(L, a, v, sh, tid, bs, nblocks, bid);
+}
+
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) k_special(DevLattice L, StepArgs a) {
+    __shared__ double sh[16][6];
+    special_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------- bulk
@@ -1025,14 +1031,16 @@ __global__ void __launch_bounds__(256) k_special(DevLattice L, StepArgs a) {
 // (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
 // xblocks pieces when nx > bx) at one z, so the +-1 x-offsets of parity 1
 // stay inside the block and waves touch whole cache lines.
+// tile bxy (of xblocks * yblocks) of z-plane z; thread tid of the tile's
+// bx * by threads (x fastest); threads past the tile idle
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
+__device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &a, uint32_t bxy, uint32_t z, uint32_t tid) {
     using T = typename C::T;
-    const uint32_t xb = blockIdx.x % L.xblocks, yb = blockIdx.x / L.xblocks;
-    const uint32_t x = xb * L.bx + threadIdx.x;
-    const uint32_t y = yb * L.by + threadIdx.y;
-    const uint32_t z = blockIdx.y;
-    const bool inside = x < L.nx && y < L.ny;
+    const uint32_t xb = bxy % L.xblocks, yb = bxy / L.xblocks;
+    const uint32_t tx = tid % L.bx, ty = tid / L.bx;
+    const uint32_t x = xb * L.bx + tx;
+    const uint32_t y = yb * L.by + ty;
+    const bool inside = x < L.nx && y < L.ny && ty < L.by;
     const uint32_t n = (z * L.ny + y) * L.nx + x;
     const uint32_t fg = inside ? L.flags[n] : 1u;
     // ours: fluid cells without a record (plain or halfway wall cells);
@@ -1055,6 +1063,41 @@ __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
         if (src) s = face_source(L, f);
         face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
     }
+}
+
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
+    bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, blockIdx.y, threadIdx.y * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- batches
+// Several independent lattices in one launch per kernel and step: block b
+// of the launch belongs to lattice i with boff[i] <= b < boff[i + 1]; the
+// lattices' device views are read from `lats` (the index is uniform per
+// block, so they come through the scalar cache). All lattices of a batch
+// share storage, forcing/sponge flags and the step index (same seed).
+__device__ __forceinline__ uint32_t batch_member(const uint32_t *boff, uint32_t nl, uint32_t b) {
+    uint32_t i = 0;
+    while (i + 1 < nl && b >= boff[i + 1]) ++i;
+    return __builtin_amdgcn_readfirstlane(i);   // uniform: keeps the lattice's fields in scalar registers
+}
+
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(1024) k_bulk_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ boff,
+                                                     uint32_t nl, StepArgs a) {
+    const uint32_t i = batch_member(boff, nl, blockIdx.x);
+    const DevLattice &L = lats[i];
+    const uint32_t local = __builtin_amdgcn_readfirstlane(blockIdx.x - boff[i]);
+    const uint32_t tiles = L.xblocks * L.yblocks;
+    bulk_block<C, P, FORCE, SPONGE>(L, a, local % tiles, local / tiles, threadIdx.x);
+}
+
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) k_special_batch(const DevLattice *__restrict__ lats,
+                                                                 const uint32_t *__restrict__ boff, uint32_t nl, StepArgs a) {
+    __shared__ double sh[16][6];
+    const uint32_t i = batch_member(boff, nl, blockIdx.x);
+    special_block<C, P, FORCE, SPONGE>(lats[i], a, blockIdx.x - boff[i], sh);
 }
 
 // ---------------------------------------------------------------- partial bounce-back and its adjoint
@@ -1686,6 +1729,36 @@ hipError_t launch_adjoint_step(const DevLattice &L, const DevLattice &A, int p, 
     }
 #undef ADJ
     return hipGetLastError();
+}
+
+namespace {
+template <class C, int P, bool FORCE, bool SPONGE>
+hipError_t batch_launch4(const DevLattice *lats, const uint32_t *boff_bulk, uint32_t nbulk, const uint32_t *boff_sp,
+                         uint32_t nsp, uint32_t nl, uint32_t threads, const StepArgs &a, hipStream_t s) {
+    k_special_batch<C, P, FORCE, SPONGE><<<nsp, SPECIAL_BLOCK, 0, s>>>(lats, boff_sp, nl, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_bulk_batch<C, P, FORCE, SPONGE><<<nbulk, threads, 0, s>>>(lats, boff_bulk, nl, a);
+    return hipGetLastError();
+}
+template <class C, int P>
+hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s) {
+    if (!b.force && !b.sponge) return batch_launch4<C, P, false, false>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
+    if (b.force && !b.sponge) return batch_launch4<C, P, true, false>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
+    if (!b.force && b.sponge) return batch_launch4<C, P, false, true>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
+    return batch_launch4<C, P, true, true>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
+}
+}  // namespace
+
+hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hipStream_t s) {
+#define BATCH(C) return p == 0 ? batch_launch2<C, 0>(b, a, s) : batch_launch2<C, 1>(b, a, s)
+    switch (b.storage) {
+        case SIMLAB_STORAGE_FP32: BATCH(CF32);
+        case SIMLAB_STORAGE_FP16: BATCH(CF16);
+        case SIMLAB_STORAGE_BF16: BATCH(CBF16);
+        default: BATCH(CF64);
+    }
+#undef BATCH
 }
 
 }  // namespace simlab

@@ -14,6 +14,9 @@
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -178,6 +181,14 @@ void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) 
     L.wall_blocks = std::max<uint32_t>(1, (L.nrec + SPECIAL_BLOCK - 1) / SPECIAL_BLOCK);
 }
 
+// an explicit tile of bx x-cells (a multiple of 64) by `by` rows
+void set_shape(DevLattice &L, uint32_t bx, uint32_t by) {
+    L.bx = bx;
+    L.by = by;
+    L.xblocks = (L.nx + bx - 1) / bx;
+    L.yblocks = (L.ny + by - 1) / by;
+}
+
 uint32_t default_block_threads() {
     const char *e = std::getenv("SIMLAB_CFD_BLOCK");
     if (e) {
@@ -271,7 +282,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     lat->has_solid = geom_kind == SIMLAB_GEOM_STATIC && g->nsolid > 0;
 
     // Wall cells whose cut links all sit halfway (q = 1/2) and lead into
-    // cells that are not face cells need no record: the bulk kernel bounces
+    // solid cells need no record: the bulk kernel bounces
     // them back from the slot the cell's own last output went to (static
     // geometry, no wall model, no sponge). Every other wall cell keeps a
     // record (record_words, lattice.rs:202-218), SoA.
@@ -294,7 +305,10 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
                     const uint32_t tx = (uint32_t)(((int)cx + CV[i][0] + (int)nx) % (int)nx);
                     const uint32_t ty = (uint32_t)(((int)cy + CV[i][1] + (int)ny) % (int)ny);
                     const uint32_t tz = (uint32_t)(((int)cz + CV[i][2] + (int)nz) % (int)nz);
-                    if (face_type(tx, ty, tz) >= 0) hw = false;
+                    // the slot read back belongs to the cell across the link:
+                    // it must be solid (a fluid cell across a thin part writes
+                    // that slot in the same step; a face cell's is rewritten)
+                    if (h_flags[((uint64_t)tz * ny + ty) * nx + tx] != 1u) hw = false;
                 }
             }
             if (hw) {
@@ -832,6 +846,243 @@ int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms, double
     if (main_ms) *main_ms = lat->last_main_ms;
     if (main_launches) *main_launches = lat->last_main_launches;
     return SIMLAB_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+}  // extern "C"
+namespace {
+
+struct TuneKey {
+    uint32_t nx, ny, nz;
+    int32_t storage;
+    bool walls;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(nx, ny, nz, storage, walls) < std::tie(o.nx, o.ny, o.nz, o.storage, o.walls);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<uint32_t, uint32_t>> g_tune;   // best (bx, by) per lattice size
+
+// tiles to try: x extents of a whole row rounded to a wave and its halves
+// (>= 64), rows per tile so a tile has 64 .. 1024 threads
+std::vector<std::pair<uint32_t, uint32_t>> tile_candidates(const DevLattice &L) {
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    uint32_t row = std::min<uint32_t>(1024, ((L.nx + 63) / 64) * 64);
+    for (uint32_t bx = row; bx >= 64; bx /= 2) {
+        if (bx % 64) break;
+        for (uint32_t t = 64; t <= 1024; t *= 2) {
+            if (t < bx) continue;
+            const uint32_t by = std::min<uint32_t>(t / bx, L.ny);
+            if (std::find(out.begin(), out.end(), std::make_pair(bx, by)) == out.end()) out.push_back({bx, by});
+        }
+        if (bx == 64) break;
+    }
+    return out;
+}
+
+}  // namespace
+extern "C" {
+
+int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cache, uint32_t *tile_out, double *step_us_out) {
+    return guarded([&]() -> int {
+        if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+        if (steps == 0) steps = 20;
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            DevLattice &L = lat->L;
+            const TuneKey key{L.nx, L.ny, L.nz, L.storage, lat->nrec != 0};
+            if (use_cache) {
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                auto it = g_tune.find(key);
+                if (it != g_tune.end()) {
+                    set_shape(L, it->second.first, it->second.second);
+                    if (tile_out) { tile_out[0] = L.bx; tile_out[1] = L.by; }
+                    if (step_us_out) *step_us_out = 0.0;
+                    return SIMLAB_OK;
+                }
+            }
+            // the trial steps advance the state: keep it and put it back
+            hipStream_t st = lat->stream;
+            const size_t pb = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            DevBuf save_pops, save_fpost, save_mac, save_ft;
+            uint64_t mem = 0;
+            save_pops.alloc(pb, mem);
+            save_fpost.alloc(lat->rfpost.bytes, mem);
+            save_mac.alloc(lat->rmac.bytes, mem);
+            save_ft.alloc(lat->rft.bytes, mem);
+            auto copy = [&](void *d, const void *s2, size_t b) {
+                hip_check(hipMemcpyAsync(d, s2, b, hipMemcpyDeviceToDevice, st), "copy");
+            };
+            copy(save_pops.p, lat->pops.p, pb);
+            copy(save_fpost.p, lat->rfpost.p, lat->rfpost.bytes);
+            copy(save_mac.p, lat->rmac.p, lat->rmac.bytes);
+            copy(save_ft.p, lat->rft.p, lat->rft.bytes);
+            const uint64_t steps0 = lat->steps_done;
+            const bool timing0 = lat->timing;
+            lat->timing = true;
+            double best = 1e30;
+            std::pair<uint32_t, uint32_t> best_tile{L.bx, L.by};
+            for (const auto &c : tile_candidates(L)) {
+                set_shape(L, c.first, c.second);
+                if (run_steps(lat, nullptr, 2, nullptr, true) != SIMLAB_OK) continue;   // warm-up
+                if (run_steps(lat, nullptr, steps, nullptr, true) != SIMLAB_OK) continue;
+                const double us = lat->last_step_ms * 1e3 / steps;
+                if (us < best) {
+                    best = us;
+                    best_tile = c;
+                }
+                copy(lat->pops.p, save_pops.p, pb);
+                copy(lat->rfpost.p, save_fpost.p, lat->rfpost.bytes);
+                copy(lat->rmac.p, save_mac.p, lat->rmac.bytes);
+                copy(lat->rft.p, save_ft.p, lat->rft.bytes);
+                lat->steps_done = steps0;
+            }
+            hip_check(hipStreamSynchronize(st), "GPU wait failed");
+            lat->timing = timing0;
+            set_shape(L, best_tile.first, best_tile.second);
+            {
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                g_tune[key] = best_tile;
+            }
+            if (tile_out) { tile_out[0] = L.bx; tile_out[1] = L.by; }
+            if (step_us_out) *step_us_out = best;
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_set_tile(simlab_lattice *lat, uint32_t bx, uint32_t by) {
+    if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+    if (bx == 0 || bx % 64 || bx > 1024 || by == 0 || bx * by > 1024)
+        return fail(SIMLAB_ERR_INVALID, "tile must be a multiple of 64 cells wide and at most 1024 threads");
+    set_shape(lat->L, bx, std::min<uint32_t>(by, lat->L.ny));
+    return SIMLAB_OK;
+}
+
+// ---------------------------------------------------------------- batches
+struct simlab_batch {
+    std::vector<simlab_lattice *> lats;
+    DevBuf dlats, boffs;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    ~simlab_batch() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+int simlab_cfd_batch_new(simlab_lattice *const *lats, size_t n, simlab_batch **out) {
+    return guarded([&]() -> int {
+        if (!lats || !out || n == 0) return fail(SIMLAB_ERR_INVALID, "empty batch");
+        *out = nullptr;
+        for (size_t i = 0; i < n; ++i) {
+            if (!lats[i]) return fail(SIMLAB_ERR_INVALID, "null lattice in batch");
+            const simlab_lattice &a = *lats[i], &b = *lats[0];
+            if (a.storage != b.storage) return fail(SIMLAB_ERR_INVALID, "a batch needs one population storage");
+            if (a.L.forcing != b.L.forcing || a.L.sponge != b.L.sponge)
+                return fail(SIMLAB_ERR_INVALID, "a batch needs the same body-force and sponge settings");
+            if (a.device != b.device) return fail(SIMLAB_ERR_INVALID, "a batch lives on one GPU");
+            if (a.moving) return fail(SIMLAB_ERR_INVALID, "a batch takes static geometry only");
+            for (size_t j = 0; j < i; ++j)
+                if (lats[j] == lats[i]) return fail(SIMLAB_ERR_INVALID, "a lattice appears twice in the batch");
+        }
+        try {
+            auto b = std::make_unique<simlab_batch>();
+            b->lats.assign(lats, lats + n);
+            b->device = lats[0]->device;
+            hip_check(hipSetDevice(b->device), "hipSetDevice");
+            uint64_t mem = 0;
+            b->dlats.alloc(n * sizeof(DevLattice), mem);
+            b->boffs.alloc(2 * (n + 1) * 4, mem);
+            hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+            *out = b.release();
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+void simlab_cfd_batch_free(simlab_batch *b) { delete b; }
+
+int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
+    return guarded([&]() -> int {
+        if (!b) return fail(SIMLAB_ERR_INVALID, "null batch");
+        const size_t n = b->lats.size();
+        for (size_t i = 1; i < n; ++i)
+            if (b->lats[i]->steps_done != b->lats[0]->steps_done)
+                return fail(SIMLAB_ERR_INVALID, "the lattices of a batch must have done the same number of steps");
+        try {
+            hip_check(hipSetDevice(b->device), "hipSetDevice");
+            hipStream_t st = b->stream;
+            // device views and block offsets as the lattices stand now
+            std::vector<DevLattice> hl(n);
+            std::vector<uint32_t> boff(2 * (n + 1), 0);
+            uint32_t threads = 64;
+            for (size_t i = 0; i < n; ++i) {
+                hip_check(hipStreamSynchronize(b->lats[i]->stream), "GPU wait failed");
+                hl[i] = b->lats[i]->L;
+                threads = std::max<uint32_t>(threads, hl[i].bx * hl[i].by);
+                boff[i + 1] = boff[i] + hl[i].xblocks * hl[i].yblocks * hl[i].nz;
+                boff[n + 1 + i + 1] = boff[n + 1 + i] + hl[i].wall_blocks + hl[i].hw_blocks;
+            }
+            hip_check(hipMemcpyAsync(b->dlats.p, hl.data(), n * sizeof(DevLattice), hipMemcpyHostToDevice, st), "upload");
+            hip_check(hipMemcpyAsync(b->boffs.p, boff.data(), boff.size() * 4, hipMemcpyHostToDevice, st), "upload");
+            BatchLaunch bl;
+            bl.lats = b->dlats.as<DevLattice>();
+            bl.boff_bulk = b->boffs.as<uint32_t>();
+            bl.boff_sp = b->boffs.as<uint32_t>() + (n + 1);
+            bl.nl = (uint32_t)n;
+            bl.nbulk = boff[n];
+            bl.nsp = boff[n + 1 + n];
+            bl.threads = threads;
+            bl.storage = hl[0].storage;
+            bl.force = hl[0].forcing != 0;
+            bl.sponge = hl[0].sponge != 0;
+            simlab_step_in idn{};
+            for (int k = 0; k < 3; ++k) idn.rot[k][k] = idn.rot_prev[k][k] = 1.0;
+            const uint64_t step0 = b->lats[0]->steps_done;
+            size_t done = 0;
+            while (done < nsteps) {
+                const size_t chunk = std::min<size_t>(KMAX, nsteps - done);
+                for (size_t k = 0; k < chunk; ++k) {
+                    const uint64_t step = step0 + done + k;
+                    const int par = (int)(step % 2);
+                    StepArgs a;
+                    pack_step(idn, step, (uint32_t)k, a);
+                    if (done + k == 0)
+                        for (size_t i = 0; i < n; ++i) {
+                            // as at the start of a lattice's own run: this step's
+                            // face inputs, then the next step's
+                            hip_check(launch_faces(hl[i], par, a.seed, st), "face kernel");
+                            hip_check(launch_faces(hl[i], 1 - par, (uint32_t)(step + 1) * 2654435761u, st), "face kernel");
+                        }
+                    hip_check(launch_batch_step(bl, par, a, st), "batched stream-collide kernels");
+                }
+                for (size_t i = 0; i < n; ++i) {
+                    simlab_lattice *l = b->lats[i];
+                    hip_check(launch_reduce_hist(hl[i], (uint32_t)chunk, st), "force reduction kernel");
+                    hip_check(hipMemcpyAsync(l->hist_host, l->hist.p, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st),
+                              "force copy");
+                }
+                hip_check(hipStreamSynchronize(st), "GPU wait failed");
+                if (ft_out)
+                    for (size_t i = 0; i < n; ++i)
+                        for (size_t k = 0; k < chunk; ++k)
+                            for (int j = 0; j < 6; ++j)
+                                ft_out[(i * nsteps + done + k) * 6 + j] = b->lats[i]->hist_host[k * HIST_STRIDE + j];
+                done += chunk;
+            }
+            for (size_t i = 0; i < n; ++i) b->lats[i]->steps_done += nsteps;
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
 }
 
 // ---------------------------------------------------------------- partial bounce-back + adjoint

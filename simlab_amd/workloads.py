@@ -7,7 +7,7 @@ from __future__ import annotations
 import math
 
 from . import FaceBc, Geometry, LatticeSetup, Storage, voxelize_static
-from .shapes import cuboid, merge, sphere
+from .shapes import cuboid, cylinder, merge, sphere
 
 
 def static_geometry(mesh, dims) -> Geometry:
@@ -120,3 +120,42 @@ def porous_box(dims=(128, 64, 64), box=24, x0=40.0, u=0.04, re=200.0, seed=20261
     lo = [int(round(x0 - box / 2)), ny // 2 - box // 2, nz // 2 - box // 2]
     phi[lo[2]:lo[2] + box, lo[1]:lo[1] + box, lo[0]:lo[0] + box] = rng.random((box, box, box), dtype=np.float32)
     return setup, phi.reshape(-1), dict(nu=nu, tau0=0.5 + 3.0 * nu, box_lo=tuple(lo), box=box, u=u, re=re, seed=seed)
+
+
+def drone_arm_mesh(scale: float = 1.0, center=(0.0, 0.0, 0.0), seed: int = 20261203):
+    """Synthetic drone-frame arm: a box hub of 40 x 40 x 12 cells (x, y, z)
+    joined to 4 cylinders (radius 6-12 cells, 90 cells long) whose axes lie
+    within 30 degrees of the y-axis, two toward +y and two toward -y; seeded,
+    times `scale`. Overlaps count as one solid (non-zero winding)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cx, cy, cz = center
+    s = scale
+    parts = [cuboid([cx - 20 * s, cy - 20 * s, cz - 6 * s], [cx + 20 * s, cy + 20 * s, cz + 6 * s])]
+    for k in range(4):
+        sign = 1.0 if k < 2 else -1.0
+        tilt = math.radians(rng.uniform(0.0, 30.0))
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        d = np.array([math.sin(tilt) * math.cos(az), sign * math.cos(tilt), math.sin(tilt) * math.sin(az)])
+        r = rng.uniform(6.0, 12.0) * s
+        off = np.array([(-1) ** k * 10.0 * s, 0.0, 0.0])
+        p0 = np.array([cx, cy, cz]) + off
+        p1 = p0 + 90.0 * s * d
+        parts.append(cylinder(list(p0), list(p1), r, 32))
+    return merge(*parts)
+
+
+def drone_arm(dims=(512, 256, 256), u=0.06, cs=0.16, re=2000.0, storage: Storage = Storage.FP16, scale=None,
+              seed: int = 20261203, debug_dump=False):
+    """The drone-arm tunnel: the arm (scaled with the domain's y size unless
+    `scale` is given) at (0.3 nx, ny/2, nz/2), velocity inlet u at -x, open
+    outlet at +x, open sides; Smagorinsky Cs; nu from Re on the hub width
+    (nu = u * 40 * scale / Re)."""
+    nx, ny, nz = dims
+    s = (ny / 256.0) if scale is None else scale
+    mesh = drone_arm_mesh(s, (0.3 * nx, ny / 2.0, nz / 2.0), seed)
+    nu = u * 40.0 * s / re
+    setup = LatticeSetup(dims=dims, faces=(FaceBc.VELOCITY, FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN),
+                         nu=nu, smagorinsky=cs, u_inf=(u, 0.0, 0.0), u_init=(u, 0.0, 0.0), storage=storage,
+                         center=(0.3 * nx, ny / 2.0, nz / 2.0), debug_dump=debug_dump)
+    return setup, static_geometry(mesh, dims), dict(nu=nu, tau0=0.5 + 3.0 * nu, scale=s, u=u, re=re, seed=seed)

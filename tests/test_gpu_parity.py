@@ -245,3 +245,33 @@ def test_macros_needs_debug_dump():
     lat = sa.Lattice(setup)
     with pytest.raises(sa.SimlabError, match="macros\\(\\) needs debug_dump"):
         lat.macros()
+
+
+def test_thin_plate_halfway_links_are_race_free(monkeypatch):
+    # A plate thinner than a cell, centred at x = 10.5: the links that cross
+    # it have q = 1/2 from both sides and lead into fluid cells. Such cells
+    # keep wall records (the record-free halfway path would read back a slot
+    # that the fluid cell across the plate writes in the same step), so the
+    # result cannot depend on how the two kernels of a step overlap.
+    from simlab_amd.shapes import cuboid
+    dims = (24, 12, 12)
+    setup = sa.LatticeSetup(dims=dims, faces=(sa.FaceBc.VELOCITY, sa.FaceBc.OPEN) + (sa.FaceBc.PERIODIC,) * 4,
+                            nu=0.05, u_inf=(0.05, 0.0, 0.0), u_init=(0.05, 0.0, 0.0), storage=sa.Storage.FP32)
+    geom = W.static_geometry(cuboid([10.4999, 2.0, 2.0], [10.5001, 10.0, 10.0]), dims)
+    say("case thin_plate", dims, "walls", len(geom.vox.walls))
+    assert len(geom.vox.walls) > 0
+    lat = sa.Lattice(setup, geom)
+    assert lat.halfway_cells == 0       # every cut link leads into a fluid cell
+    monkeypatch.setenv("SIMLAB_CFD_SERIAL", "1")
+    ser = sa.Lattice(setup, geom)
+    monkeypatch.delenv("SIMLAB_CFD_SERIAL")
+    for _ in range(3):
+        lat.run_still(4)
+        ser.run_still(4)
+    assert np.array_equal(lat.get_pops(), ser.get_pops())
+    ora = O.OracleLattice(setup, geom, store=O.F32)
+    g1 = sa.Lattice(setup, geom)
+    g1.run_still(1)
+    ora.run_still(1)
+    d = np.abs(live_post(g1.get_pops(), ora, 0).astype(np.float64) - live_post(ora.get_pops(), ora, 0).astype(np.float64))
+    assert d.max() < 2e-7
