@@ -293,6 +293,19 @@ int simlab_cfd_voxels_view(const simlab_static_voxels *v, uint64_t *nsolid,
                            uint64_t hi[3]);
 void simlab_cfd_voxels_free(simlab_static_voxels *v);
 
+/* `voxelize_spin` (voxel.rs:509-553) on the host CPU: a spinning part given
+ * in body coordinates (lattice units, origin at the spin centre): occupancy
+ * bits at spacing h (x fastest) and triangles split to edges <= 1.5 cells,
+ * stored as (v0, e1, e2) float x 12; `axis` is the spin axis for the radius
+ * and axial extent. The views stay valid until the object is freed. */
+typedef struct simlab_spin_voxels simlab_spin_voxels;   /* opaque */
+int simlab_cfd_voxelize_spin(const double *verts, uint64_t nverts, const uint32_t *tris, uint64_t ntris,
+                             const double axis[3], double h, simlab_spin_voxels **out);
+int simlab_cfd_spin_voxels_view(const simlab_spin_voxels *v, double occ_lo[3], double *occ_h, uint64_t occ_dims[3],
+                                uint64_t *nocc_words, const uint32_t **occ_bits, uint64_t *ntri, const float **tris,
+                                double *radius, double axial[2]);
+void simlab_cfd_spin_voxels_free(simlab_spin_voxels *v);
+
 /* `inside_grid` (voxel.rs:73-95): inside flags (0/1) for the grid points
  * origin + h * (i, j, k), i fastest. out: dims[0]*dims[1]*dims[2] bytes. */
 int simlab_cfd_inside_grid(const double *verts, uint64_t nverts,

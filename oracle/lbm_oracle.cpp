@@ -610,7 +610,7 @@ template <class T> struct Lat {
         if (b < 0.0f || a + b > 1.0f) return 9.0f;
         float t = dot(e2, qv) * inv;
         if (t < -1e-4f || t > 1.0f) return 9.0f;
-        return std::max(t, 0.0f);
+        return t > 0.0f ? t : 0.0f;   // max(t, 0) as +0 for t = -0: hit bits must sort as the floats do
     }
     void tri_scatter(const StepU &s) {
         size_t ntri = geo.tris.size() / 12;
@@ -646,7 +646,7 @@ template <class T> struct Lat {
                             float tt = ray_tri(pp, ci, a, e1, e2);
                             if (tt <= 1.0f) {
                                 size_t o = ((size_t)k * 9 + (i - 1) / 2) * 3;
-                                uint32_t tb = fbits(std::max(tt, 0.0f));
+                                uint32_t tb = fbits(tt);   // tt >= +0 (ray_tri)
                                 qscr[o] = std::min(qscr[o], tb);
                                 qscr[o + 1] = std::max(qscr[o + 1], tb);
                                 qscr[o + 2] += 1;

@@ -23,7 +23,7 @@ import numpy as np
 __all__ = [
     "SimlabError", "Storage", "FaceBc", "LatticeSetup", "StepIn", "IDENTITY", "Geometry", "Probes",
     "SliceDef", "Lattice", "StaticVoxels", "voxelize_static", "inside_grid", "gpu_info", "max_cells",
-    "load_library", "LIB_PATH", "Batch",
+    "load_library", "LIB_PATH", "Batch", "SpinVoxels", "voxelize_spin",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsimlab_cfd.so")
@@ -128,6 +128,10 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_adjoint_forward": (I, [V, C.c_size_t, C.c_size_t, C.c_size_t, I, C.POINTER(D), P]),
         "simlab_cfd_adjoint_backward": (I, [V, P]),
         "simlab_cfd_voxelize_static": (I, [P, U64, P, U64, C.POINTER(U64), C.POINTER(V)]),
+        "simlab_cfd_voxelize_spin": (I, [P, U64, P, U64, C.POINTER(D), D, C.POINTER(V)]),
+        "simlab_cfd_spin_voxels_view": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64), C.POINTER(U64), C.POINTER(P),
+                                            C.POINTER(U64), C.POINTER(P), C.POINTER(D), C.POINTER(D)]),
+        "simlab_cfd_spin_voxels_free": (None, [V]),
         "simlab_cfd_voxels_view": (I, [V, C.POINTER(U64), C.POINTER(P), C.POINTER(U64), C.POINTER(P),
                                        C.POINTER(U64), C.POINTER(U64)]),
         "simlab_cfd_voxels_free": (None, [V]),
@@ -249,13 +253,35 @@ class StaticVoxels:
 
 @dataclass
 class Geometry:
-    """lattice.rs:87-92 (None or Static here; Spin where the build has it)."""
+    """lattice.rs:87-92: None, Static (solid cells + wall links) or Spin (a
+    spinning part's body-frame occupancy and triangles, re-voxelized every
+    step inside the swept box box_lo .. box_lo + box_n)."""
     kind: str = "none"
-    vox: Optional[StaticVoxels] = None
+    vox: Optional[object] = None
+    box_lo: tuple = (0, 0, 0)
+    box_n: tuple = (0, 0, 0)
 
     @staticmethod
     def static(vox: StaticVoxels) -> "Geometry":
         return Geometry("static", vox)
+
+    @staticmethod
+    def spin(vox: "SpinVoxels", box_lo, box_n) -> "Geometry":
+        return Geometry("spin", vox, tuple(int(v) for v in box_lo), tuple(int(v) for v in box_n))
+
+
+@dataclass
+class SpinVoxels:
+    """voxel.rs:492-505: body-frame occupancy bits at spacing occ_h from
+    occ_lo (x fastest), triangles (v0, e1, e2; ntri x 12 float32, body
+    frame), largest distance from the spin axis and the axial extent."""
+    occ_lo: tuple
+    occ_h: float
+    occ_dims: tuple
+    occ_bits: np.ndarray
+    tris: np.ndarray
+    radius: float
+    axial: tuple
 
 
 @dataclass
@@ -320,6 +346,34 @@ def voxelize_static(verts, tris, dims) -> StaticVoxels:
         lib.simlab_cfd_voxels_free(h)
 
 
+def voxelize_spin(verts, tris, axis, h: float) -> SpinVoxels:
+    """voxel.rs:509-553: a spinning part in body coordinates (lattice units,
+    origin at the spin centre): occupancy at spacing h, triangles split to
+    edges of at most 1.5 cells."""
+    lib = load_library()
+    v, t = _surface_arrays(verts, tris)
+    ax = (C.c_double * 3)(*[float(a) for a in axis])
+    hnd = C.c_void_p()
+    _check(lib, lib.simlab_cfd_voxelize_spin(_ptr(v), len(v), _ptr(t), len(t), ax, float(h), C.byref(hnd)))
+    try:
+        lo = (C.c_double * 3)()
+        oh = C.c_double()
+        dims = (C.c_uint64 * 3)()
+        nw, nt = C.c_uint64(), C.c_uint64()
+        bp, tp = C.c_void_p(), C.c_void_p()
+        rad = C.c_double()
+        axial = (C.c_double * 2)()
+        _check(lib, lib.simlab_cfd_spin_voxels_view(hnd, lo, C.byref(oh), dims, C.byref(nw), C.byref(bp), C.byref(nt),
+                                                    C.byref(tp), C.byref(rad), axial))
+        bits = np.ctypeslib.as_array(C.cast(bp, C.POINTER(C.c_uint32)), shape=(nw.value,)).copy() if nw.value else \
+            np.zeros(0, np.uint32)
+        tr = np.ctypeslib.as_array(C.cast(tp, C.POINTER(C.c_float)), shape=(nt.value * 12,)).copy().reshape(-1, 12) \
+            if nt.value else np.zeros((0, 12), np.float32)
+        return SpinVoxels(tuple(lo), oh.value, tuple(int(d) for d in dims), bits, tr, rad.value, (axial[0], axial[1]))
+    finally:
+        lib.simlab_cfd_spin_voxels_free(hnd)
+
+
 def inside_grid(verts, tris, origin, h, dims) -> np.ndarray:
     """voxel.rs:73-95: inside flags of the grid points origin + h*(i,j,k), i fastest."""
     lib = load_library()
@@ -349,6 +403,22 @@ def geometry_to_c(geom: Optional[Geometry]):
         g.solid = solid.ctypes.data_as(C.POINTER(C.c_uint32))
         g.nwall = len(walls)
         g.walls = walls.ctypes.data_as(C.c_void_p)
+        return g, keep
+    if geom.kind == "spin":
+        v = geom.vox
+        bits = np.ascontiguousarray(v.occ_bits, dtype=np.uint32)
+        tris = np.ascontiguousarray(v.tris, dtype=np.float32).reshape(-1, 12)
+        keep += [bits, tris]
+        g.kind = 2
+        g.occ_lo[:] = [float(x) for x in v.occ_lo]
+        g.occ_h = float(v.occ_h)
+        g.occ_dims[:] = [int(x) for x in v.occ_dims]
+        g.nocc_words = len(bits)
+        g.occ_bits = bits.ctypes.data_as(C.POINTER(C.c_uint32))
+        g.ntri = len(tris)
+        g.tris = tris.ctypes.data_as(C.POINTER(C.c_float))
+        g.box_lo[:] = list(geom.box_lo)
+        g.box_n[:] = list(geom.box_n)
         return g, keep
     raise SimlabError(f"unknown geometry kind {geom.kind!r}")
 

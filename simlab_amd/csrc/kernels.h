@@ -111,13 +111,15 @@ hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStrea
 hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
+// re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)
 hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+// resets the link scratch of a spinning part (setup)
+hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
 hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 // force history of the first nsteps steps of a chunk from the partials
 hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t s);
 // the same over an explicit partials array ([nsteps][nparts][8] -> hist)
 hipError_t launch_reduce_partials(const double *partials, uint32_t nparts, uint32_t nsteps, double *hist, hipStream_t s);
-hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
 // partial bounce-back: one forward step (FP32/FP64 storage), and the adjoint
 // of one step (A.pops: adjoint populations in the storage's type; grad: d/d phi)
 hipError_t launch_pbb_step(const DevLattice &L, int parity, const StepArgs &a, const float *phi, hipStream_t s);

@@ -1100,6 +1100,300 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_special_batch(const DevLattic
     special_block<C, P, FORCE, SPONGE>(lats[i], a, blockIdx.x - boff[i], sh);
 }
 
+// ---------------------------------------------------------------- moving geometry
+// Re-voxelization of a spinning part's swept box every step (shader.rs:865-1104):
+// tri_scatter finds the lattice links each rotated triangle cuts, voxelize
+// classifies every box cell (solid now / before from the body-frame
+// occupancy bits), builds its wall links and q, refills cells the part
+// uncovered and books the momentum of covered and uncovered cells, qclear
+// resets the link scratch. Body-frame arithmetic is in f32 as in the WGSL,
+// with no fused multiply-adds, so every lattice finds the same links and q
+// bits for the same rotation whatever the compiler schedules.
+struct V3f {
+    float x, y, z;
+};
+__device__ __forceinline__ V3f v3(float x, float y, float z) { return V3f{x, y, z}; }
+__device__ __forceinline__ V3f operator+(V3f a, V3f b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3f operator-(V3f a, V3f b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot3(V3f a, V3f b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ V3f cross3(V3f a, V3f b) {
+#pragma clang fp contract(off)
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// rotation columns r[k] = body axis k: to_body(v) = (r0.v, r1.v, r2.v)
+__device__ __forceinline__ V3f to_body(const float (*m)[4], V3f v) {
+#pragma clang fp contract(off)
+    return v3(m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+              m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z);
+}
+__device__ __forceinline__ V3f from_body(const float (*m)[4], V3f b) {
+#pragma clang fp contract(off)
+    return v3(m[0][0] * b.x + m[1][0] * b.y + m[2][0] * b.z, m[0][1] * b.x + m[1][1] * b.y + m[2][1] * b.z,
+              m[0][2] * b.x + m[1][2] * b.y + m[2][2] * b.z);
+}
+__device__ __forceinline__ bool occ_at(const DevLattice &L, V3f b) {
+#pragma clang fp contract(off)
+    const int gx = (int)__builtin_rintf((b.x - L.occ_lo[0]) * L.occ_invh);
+    const int gy = (int)__builtin_rintf((b.y - L.occ_lo[1]) * L.occ_invh);
+    const int gz = (int)__builtin_rintf((b.z - L.occ_lo[2]) * L.occ_invh);
+    if (gx < 0 || gy < 0 || gz < 0 || gx >= L.occ_n[0] || gy >= L.occ_n[1] || gz >= L.occ_n[2]) return false;
+    const uint32_t l = (uint32_t)((gz * L.occ_n[1] + gy) * L.occ_n[0] + gx);
+    return ((L.occ[l >> 5] >> (l & 31u)) & 1u) != 0u;
+}
+__device__ __forceinline__ V3f centerf(const DevLattice &L) { return v3((float)L.center[0], (float)L.center[1], (float)L.center[2]); }
+// every cell asks with the same arithmetic for itself and its neighbours
+__device__ __forceinline__ bool solid_at(const DevLattice &L, const StepArgs &a, int px, int py, int pz, bool before) {
+    const V3f c = centerf(L);
+    const V3f pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
+    return occ_at(L, to_body(before ? a.q : a.r, pc));
+}
+// Moller-Trumbore for the segment o + t d, t in [0, 1]; 9 for no hit
+__device__ __forceinline__ float ray_tri(V3f o, V3f d, V3f v0, V3f e1, V3f e2) {
+#pragma clang fp contract(off)
+    const V3f pv = cross3(d, e2);
+    const float det = dot3(e1, pv);
+    if (__builtin_fabsf(det) < 1e-12f) return 9.0f;
+    const float inv = 1.0f / det;
+    const V3f tv = o - v0;
+    const float u = dot3(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return 9.0f;
+    const V3f qv = cross3(tv, e1);
+    const float v = dot3(d, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 9.0f;
+    const float t = dot3(e2, qv) * inv;
+    if (t < -1e-4f || t > 1.0f) return 9.0f;
+    // max(t, 0), with +0 for t = -0: the hit bits are ordered as unsigned
+    // integers, which sort non-negative floats only
+    return t > 0.0f ? t : 0.0f;
+}
+
+__global__ void __launch_bounds__(64) k_tri_scatter(DevLattice L, StepArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.ntri) return;
+    const float4 t0 = L.tris[3 * t], t1 = L.tris[3 * t + 1], t2 = L.tris[3 * t + 2];
+    const V3f A = centerf(L) + from_body(a.r, v3(t0.x, t0.y, t0.z));
+    const V3f e1 = from_body(a.r, v3(t1.x, t1.y, t1.z));
+    const V3f e2 = from_body(a.r, v3(t2.x, t2.y, t2.z));
+    const V3f nrm = cross3(e1, e2);
+    const float nl = __builtin_sqrtf(dot3(nrm, nrm));
+    if (nl < 1e-12f) return;
+    const V3f nn = v3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
+    const V3f B = A + e1, Cc = A + e2;
+    int lo[3], hi[3];
+    {
+        const float av[3] = {A.x, A.y, A.z}, bv[3] = {B.x, B.y, B.z}, cv[3] = {Cc.x, Cc.y, Cc.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float mn = fminf(fminf(av[k], bv[k]), cv[k]), mx = fmaxf(fmaxf(av[k], bv[k]), cv[k]);
+            lo[k] = max((int)__builtin_ceilf(mn - 1.415f), L.box_lo[k]);
+            hi[k] = min((int)__builtin_floorf(mx + 1.415f), L.box_lo[k] + (int)L.box_n[k] - 1);
+        }
+    }
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                const V3f p = v3((float)x, (float)y, (float)z);
+                const float d0 = dot3(nn, p - A);
+                if (__builtin_fabsf(d0) > 1.415f) continue;
+                const uint32_t k = ((uint32_t)(z - L.box_lo[2]) * L.box_n[1] + (uint32_t)(y - L.box_lo[1])) * L.box_n[0] +
+                                   (uint32_t)(x - L.box_lo[0]);
+                for (int i = 1; i < 19; i += 2) {
+                    const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
+                    const float d1 = d0 + dot3(nn, ci);
+                    if (d0 * d1 > 0.0f && __builtin_fabsf(d0) > 1e-6f) continue;
+                    const float tt = ray_tri(p, ci, A, e1, e2);
+                    if (tt <= 1.0f) {
+                        uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
+                        const uint32_t tb = __float_as_uint(tt);   // tt >= +0 (ray_tri)
+                        atomicMin(o, tb);
+                        atomicMax(o + 1, tb);
+                        atomicAdd(o + 2, 1u);
+                    }
+                }
+            }
+}
+
+__global__ void __launch_bounds__(256) k_qclear(DevLattice L) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L.nrec * 9u) return;
+    L.qscr[3 * (size_t)k] = 0x7f7fffffu;
+    L.qscr[3 * (size_t)k + 1] = 0u;
+    L.qscr[3 * (size_t)k + 2] = 0u;
+}
+
+// f16 round to nearest even of a q in [0, 1] (pack2x16float)
+__device__ __forceinline__ uint32_t half_rne_bits(float q) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)q);
+}
+
+// the refill's store: this step's inputs of the uncovered cell, written as
+// the cell's previous outputs (store_prev_out, guarded, per-store dither)
+template <class C, int Q>
+__device__ __forceinline__ void refill_store(const DevLattice &L, const Nbr &nb, const typename C::T fe[19], uint32_t seed) {
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        uint32_t cell, dir;
+        if constexpr (Q == 0 || i == 0) {
+            cell = nb.n;
+            dir = OPP[i];
+        } else {
+            if (!keeps_inside<i>(L, nb)) return;
+            cell = nb.template at<CX[i], CY[i], CZ[i]>();
+            dir = i;
+        }
+        const uint32_t d16 = C::SR ? (cell_hash(nb.n * 19u + dir, seed) >> 16) : 0u;
+        popdir<C>(L, dir)[cell] = enc<C>(fe[i], d16);
+    });
+}
+
+template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevLattice L, StepArgs a) {
+    using T = typename C::T;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L.nrec) return;
+    const uint32_t BX = L.box_n[0], BY = L.box_n[1], BZ = L.box_n[2];
+    const uint32_t lx = k % BX, ly = (k / BX) % BY, lz = k / (BX * BY);
+    const int px = (int)lx + L.box_lo[0], py = (int)ly + L.box_lo[1], pz = (int)lz + L.box_lo[2];
+    const uint32_t n = ((uint32_t)pz * L.ny + (uint32_t)py) * L.nx + (uint32_t)px;
+    const V3f c = centerf(L);
+    const V3f pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
+    const bool solid_now = solid_at(L, a, px, py, pz, false);
+    const bool solid_before = solid_at(L, a, px, py, pz, true);
+    const size_t NR = L.nrec;
+    T *rfpost = (T *)L.rfpost;
+    T *rmac = (T *)L.rmac;
+    T *rft = (T *)L.rft;
+    const T pcx = (T)pc.x, pcy = (T)pc.y, pcz = (T)pc.z;
+    if (solid_now) {
+        L.flags[n] = 1u | ((k + 1u) << 2);
+        L.rmask[k] = 0u;
+        T F[3] = {0, 0, 0};
+        if (!solid_before) {
+            // a cell the part covered: the part takes its fluid's momentum
+            const T rho = rmac[k];
+            F[0] = rho * rmac[NR + k];
+            F[1] = rho * rmac[2 * NR + k];
+            F[2] = rho * rmac[3 * NR + k];
+        }
+        rft[k] = F[0]; rft[NR + k] = F[1]; rft[2 * NR + k] = F[2];
+        rft[3 * NR + k] = pcy * F[2] - pcz * F[1];
+        rft[4 * NR + k] = pcz * F[0] - pcx * F[2];
+        rft[5 * NR + k] = pcx * F[1] - pcy * F[0];
+        return;
+    }
+    // link hits of this step: a link is kept at the cell it leaves in the
+    // positive direction; the cell at the other end reads it mirrored
+    float qv[19], qx[19];
+    uint32_t nh[19];
+    for (int i = 1; i < 19; ++i) {
+        qv[i] = 9.0f;
+        qx[i] = -1.0f;
+        nh[i] = 0u;
+        if (i & 1) {
+            const uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
+            const uint32_t cnt = o[2];
+            if (cnt > 0u) {
+                qv[i] = __uint_as_float(o[0]);
+                qx[i] = __uint_as_float(o[1]);
+                nh[i] = cnt;
+            }
+        } else {
+            const int qx_ = (int)lx + rcx(i), qy_ = (int)ly + rcy(i), qz_ = (int)lz + rcz(i);
+            if (qx_ >= 0 && qy_ >= 0 && qz_ >= 0 && qx_ < (int)BX && qy_ < (int)BY && qz_ < (int)BZ) {
+                const uint32_t k2 = ((uint32_t)qz_ * BY + (uint32_t)qy_) * BX + (uint32_t)qx_;
+                const uint32_t *o = L.qscr + ((size_t)k2 * 9 + (i - 2) / 2) * 3;
+                const uint32_t cnt = o[2];
+                if (cnt > 0u) {
+                    qv[i] = 1.0f - __uint_as_float(o[1]);
+                    qx[i] = 1.0f - __uint_as_float(o[0]);
+                    nh[i] = cnt;
+                }
+            }
+        }
+    }
+    uint32_t mask = 0;
+    for (int i = 1; i < 19; ++i) {
+        // a wall where the neighbour is solid, or where the link crosses a
+        // sheet thinner than a cell (two hits); q from the nearest hit
+        if (solid_at(L, a, px + rcx(i), py + rcy(i), pz + rcz(i), false)) {
+            if (nh[i] == 0u) qv[i] = 1.0f;
+            mask |= 1u << i;
+        } else if (nh[i] >= 2u && qx[i] - qv[i] > 1e-5f) {
+            mask |= 1u << i;
+        } else {
+            qv[i] = 1.0f;
+        }
+    }
+    for (int w = 0; w < 9; ++w)
+        L.rq[(size_t)w * NR + k] = half_rne_bits(fminf(qv[2 * w + 1], 1.0f)) | (half_rne_bits(fminf(qv[2 * w + 2], 1.0f)) << 16);
+    L.rmask[k] = mask;
+    T F[3] = {0, 0, 0};
+    if (solid_before) {
+        // a cell the part uncovered: refill what it would have sent out last
+        // step with the equilibrium at the wall velocity and the density of
+        // its fluid neighbours, plus their mean non-equilibrium stress
+        T rs = 0, cnt = 0;
+        T pis[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 1; i < 19; ++i) {
+            const bool fluid_now = !solid_at(L, a, px + rcx(i), py + rcy(i), pz + rcz(i), false);
+            const bool fluid_before = !solid_at(L, a, px + rcx(i), py + rcy(i), pz + rcz(i), true);
+            if (!(fluid_now && fluid_before)) continue;
+            const int qx_ = (int)lx + rcx(i), qy_ = (int)ly + rcy(i), qz_ = (int)lz + rcz(i);
+            if (qx_ < 0 || qy_ < 0 || qz_ < 0 || qx_ >= (int)BX || qy_ >= (int)BY || qz_ >= (int)BZ) continue;
+            const uint32_t k2 = ((uint32_t)qz_ * BY + (uint32_t)qy_) * BX + (uint32_t)qx_;
+            const T r2 = rmac[k2];
+            rs = rs + r2;
+            cnt = cnt + (T)1;
+            T fp[19];
+            fp[0] = (T)0;
+            for (int j = 1; j < 19; ++j) fp[j] = rfpost[(size_t)(j - 1) * NR + k2];
+            const Mom<T> mo = moments(fp);
+            const T ux = rmac[NR + k2], uy = rmac[2 * NR + k2], uz = rmac[3 * NR + k2];
+            pis[3] = pis[3] + (mo.sxy - r2 * ux * uy);
+            pis[4] = pis[4] + (mo.sxz - r2 * ux * uz);
+            pis[5] = pis[5] + (mo.syz - r2 * uy * uz);
+            const T tr = (mo.sxx + mo.syy + mo.szz) / (T)3;
+            const T uu = ux * ux + uy * uy + uz * uz;
+            pis[0] = pis[0] + (mo.sxx - tr - r2 * (ux * ux - uu / (T)3));
+            pis[1] = pis[1] + (mo.syy - tr - r2 * (uy * uy - uu / (T)3));
+            pis[2] = pis[2] + (mo.szz - tr - r2 * (uz * uz - uu / (T)3));
+        }
+        const T rho = cnt > (T)0 ? rs / cnt : (T)1;
+        const V3f uwf = cross3(v3(a.omega[0], a.omega[1], a.omega[2]), pc);
+        const T uwx = (T)uwf.x, uwy = (T)uwf.y, uwz = (T)uwf.z;
+        T fe[19];
+        feq_arr<T>(rho - (T)1, uwx, uwy, uwz, fe);
+        if (cnt > (T)0) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) pis[j] = pis[j] / cnt;
+            T fq[19];
+            fneq_arr<T>(pis, fq);
+#pragma unroll
+            for (int j = 0; j < 19; ++j) fe[j] = fe[j] + fq[j];
+        }
+        Nbr nb;
+        nb.make(L, (uint32_t)px, (uint32_t)py, (uint32_t)pz, n);
+        refill_store<C, 1 - P>(L, nb, fe, a.seed);
+        F[0] = -rho * uwx;
+        F[1] = -rho * uwy;
+        F[2] = -rho * uwz;
+        for (int i = 1; i < 19; ++i) rfpost[(size_t)(i - 1) * NR + k] = fe[i];
+        rmac[k] = rho;
+        rmac[NR + k] = uwx;
+        rmac[2 * NR + k] = uwy;
+        rmac[3 * NR + k] = uwz;
+    }
+    rft[k] = F[0]; rft[NR + k] = F[1]; rft[2 * NR + k] = F[2];
+    rft[3 * NR + k] = pcy * F[2] - pcz * F[1];
+    rft[4 * NR + k] = pcz * F[0] - pcx * F[2];
+    rft[5 * NR + k] = pcx * F[1] - pcy * F[0];
+    L.flags[n] = (k + 1u) << 2;
+}
+
 // ---------------------------------------------------------------- partial bounce-back and its adjoint
 // Partial bounce-back (new; the reference has no porous cells): a cell with
 // solid fraction phi blends its post-collision populations with the bounced
@@ -1759,6 +2053,30 @@ hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hip
         default: BATCH(CF64);
     }
 #undef BATCH
+}
+
+hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 64), 64, 0, s>>>(L, a);
+    const uint32_t nb = cdiv(L.nrec, 256);
+#define VOX(C)                                                          \
+    do {                                                                \
+        if (p == 0) k_voxelize<C, 0><<<nb, 256, 0, s>>>(L, a);          \
+        else k_voxelize<C, 1><<<nb, 256, 0, s>>>(L, a);                 \
+    } while (0)
+    switch (L.storage) {
+        case SIMLAB_STORAGE_FP32: VOX(CF32); break;
+        case SIMLAB_STORAGE_FP16: VOX(CF16); break;
+        case SIMLAB_STORAGE_BF16: VOX(CBF16); break;
+        default: VOX(CF64); break;
+    }
+#undef VOX
+    k_qclear<<<cdiv((uint64_t)L.nrec * 9, 256), 256, 0, s>>>(L);
+    return hipGetLastError();
+}
+
+hipError_t launch_qclear(const DevLattice &L, hipStream_t s) {
+    k_qclear<<<cdiv((uint64_t)L.nrec * 9, 256), 256, 0, s>>>(L);
+    return hipGetLastError();
 }
 
 }  // namespace simlab

@@ -236,7 +236,9 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         for (int k = 0; k < 3; ++k)
             if (g->box_lo[k] < 1 || (int64_t)g->box_lo[k] + g->box_n[k] > (int64_t)su->dims[k] - 1)
                 return fail(SIMLAB_ERR_INVALID, "the spinning part's swept box leaves the domain");
-        return fail(SIMLAB_ERR_INVALID, "spinning geometry is not available in this build");
+        if (!(g->occ_h > 0.0) || !g->occ_bits || g->nocc_words * 32 < (uint64_t)g->occ_dims[0] * g->occ_dims[1] * g->occ_dims[2] ||
+            (g->ntri && !g->tris))
+            return fail(SIMLAB_ERR_INVALID, "spinning geometry: occupancy bits or triangles missing");
     }
     if (geom_kind == SIMLAB_GEOM_STATIC) {
         for (uint64_t r = 0; r < g->nwall; ++r)
@@ -320,8 +322,12 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         }
     }
     lat->nhalfway = nhalfway;
-    const uint32_t nrec = (uint32_t)keep.size();
+    // a spinning part keeps one record per cell of its swept box (x fastest;
+    // lattice.rs:465-478), re-voxelized every step
+    const bool spin = geom_kind == SIMLAB_GEOM_SPIN;
+    const uint32_t nrec = spin ? (uint32_t)((uint64_t)g->box_n[0] * g->box_n[1] * g->box_n[2]) : (uint32_t)keep.size();
     lat->nrec = nrec;
+    lat->moving = spin;
     const size_t rb = real_bytes(su->storage);
     std::vector<uint32_t> h_mask(nrec), h_cell(nrec), h_q((size_t)9 * nrec);
     std::vector<unsigned char> h_fpost((size_t)18 * nrec * rb), h_mac((size_t)4 * nrec * rb), h_ft((size_t)6 * nrec * rb, 0);
@@ -342,15 +348,24 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
             }
         };
         for (uint32_t r = 0; r < nrec; ++r) {
-            const simlab_wall_cell &w = g->walls[keep[r]];
-            h_mask[r] = w.mask;
-            h_cell[r] = w.cell;
-            for (int k = 0; k < 9; ++k)
-                h_q[(size_t)k * nrec + r] = (uint32_t)f16_bits_host(w.q[2 * k + 1]) | ((uint32_t)f16_bits_host(w.q[2 * k + 2]) << 16);
+            uint32_t cell;
+            if (spin) {
+                const uint32_t bx = r % g->box_n[0], by = (r / g->box_n[0]) % g->box_n[1], bz = r / (g->box_n[0] * g->box_n[1]);
+                cell = (uint32_t)(((uint64_t)(bz + g->box_lo[2]) * ny + (by + g->box_lo[1])) * nx + bx + g->box_lo[0]);
+                h_mask[r] = 0;
+                for (int k = 0; k < 9; ++k) h_q[(size_t)k * nrec + r] = 0x3C003C00u;   // q = 1
+            } else {
+                const simlab_wall_cell &w = g->walls[keep[r]];
+                cell = w.cell;
+                h_mask[r] = w.mask;
+                for (int k = 0; k < 9; ++k)
+                    h_q[(size_t)k * nrec + r] = (uint32_t)f16_bits_host(w.q[2 * k + 1]) | ((uint32_t)f16_bits_host(w.q[2 * k + 2]) << 16);
+            }
+            h_cell[r] = cell;
             for (int i = 1; i < 19; ++i) put(h_fpost, (size_t)(i - 1) * nrec + r, fe[i]);
             put(h_mac, r, 1.0);
             for (int k = 0; k < 3; ++k) put(h_mac, (size_t)(k + 1) * nrec + r, rb == 8 ? ui[k] : (double)(float)ui[k]);
-            if ((h_flags[w.cell] & 3u) == 0) h_flags[w.cell] = (r + 1) << 2;   // record index + 1
+            if ((h_flags[cell] & 3u) == 0) h_flags[cell] = (r + 1) << 2;   // record index + 1
         }
     }
 
@@ -466,7 +481,29 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.abs_u = su->absorb_velocity != 0;
     L.total_p = su->total_pressure_inlet != 0;
     L.wall_model = su->wall_model != 0;
-    L.moving = 0;
+    L.moving = lat->moving ? 1 : 0;
+    if (lat->moving) {
+        // body-frame occupancy, triangles (v0, e1, e2) and the link scratch
+        uint64_t m2 = 0;
+        lat->occ.alloc(g->nocc_words * 4, m2);
+        lat->tris.alloc(std::max<uint64_t>(1, g->ntri) * 3 * sizeof(float4), m2);
+        lat->qscr.alloc((size_t)lat->nrec * 27 * 4, m2);
+        mem += m2;
+        hip_check(hipMemcpyAsync(lat->occ.p, g->occ_bits, g->nocc_words * 4, hipMemcpyHostToDevice, st), "upload");
+        if (g->ntri)
+            hip_check(hipMemcpyAsync(lat->tris.p, g->tris, g->ntri * 3 * sizeof(float4), hipMemcpyHostToDevice, st), "upload");
+        for (int k = 0; k < 3; ++k) {
+            L.box_lo[k] = g->box_lo[k];
+            L.box_n[k] = g->box_n[k];
+            L.occ_lo[k] = (float)g->occ_lo[k];
+            L.occ_n[k] = (int32_t)g->occ_dims[k];
+        }
+        L.occ_invh = (float)(1.0 / g->occ_h);
+        L.occ = lat->occ.as<uint32_t>();
+        L.ntri = (uint32_t)g->ntri;
+        L.tris = lat->tris.as<float4>();
+        L.qscr = lat->qscr.as<uint32_t>();
+    }
     L.forcing = (su->body_force[0] != 0.0 || su->body_force[1] != 0.0 || su->body_force[2] != 0.0) ? 1 : 0;
     L.sponge = 0;
     for (int k = 0; k < 6; ++k)
@@ -495,6 +532,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     lat->memory_bytes = mem;
 
     hip_check(launch_init(L, st), "init kernel");
+    if (lat->moving) hip_check(launch_qclear(L, st), "link scratch clear");
     hip_check(hipStreamSynchronize(st), "setup");
     *out = lat.release();
     return SIMLAB_OK;
@@ -530,6 +568,8 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             const int par = (int)(step % 2);
             StepArgs a;
             pack_step(s, step, (uint32_t)k, a);
+            // a spinning part is re-voxelized for this step's rotation first
+            if (lat->moving) hip_check(launch_spin(lat->L, par, a, st), "re-voxelization kernels");
             if (done + k == 0 && lat->fused_faces) {
                 // first step of this call: rebuild this step's face inputs from
                 // the stored state, then the next step's (constant where the

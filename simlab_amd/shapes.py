@@ -186,3 +186,126 @@ def volume(m: TriMesh) -> float:
     for t in m.triangles:
         s += _dot(m.positions[t[0]], _cross(m.positions[t[1]], m.positions[t[2]])) / 6.0
     return s
+
+
+@dataclass
+class PropSpec:
+    """shapes.rs:119-147: a propeller by radius, hub, blade count, geometric
+    pitch (advance per revolution), chord at 75 % radius and section
+    thickness / chord; same length unit throughout."""
+    radius: float
+    hub_radius: float
+    hub_height: float
+    blades: int
+    pitch: float
+    chord: float
+    thick: float
+
+    @staticmethod
+    def five_inch() -> "PropSpec":
+        """A 5-inch-class prop (5x4.3 pitch) in metres (shapes.rs:133-136)."""
+        return PropSpec(0.0635, 0.0075, 0.008, 2, 0.109, 0.0145, 0.10)
+
+    def chord_at(self, r: float) -> float:
+        x = min(max(r / self.radius, 0.0), 1.0)
+        shape = math.sqrt(max(1.0 - x * x, 0.0)) * (0.55 + 0.9 * x - 0.4 * x * x)
+        s75 = math.sqrt(1.0 - 0.5625) * (0.55 + 0.9 * 0.75 - 0.4 * 0.5625)
+        return max(self.chord * shape / s75, 0.25 * self.chord)
+
+    def pitch_angle(self, r: float) -> float:
+        return math.atan(self.pitch / (2.0 * math.pi * r))
+
+    def mesh(self) -> TriMesh:
+        """Hub cylinder plus blades lofted from sections (shapes.rs:149-230)."""
+        return _prop_mesh(self.radius, self.hub_radius, self.hub_height, self.blades, self.chord_at,
+                          self.pitch_angle, self.thick)
+
+
+def _section_point(i: int, nc: int, thick: float):
+    """shapes.rs:233-246: (chordwise s from the leading edge, thickness
+    offset / chord) around a flat-bottomed, rounded-top section."""
+    half = nc // 2
+    if i <= half:
+        s = 0.5 - 0.5 * math.cos(math.pi * i / half)
+        return s, thick * 2.6 * math.sqrt(s) * (1.0 - s) * 0.77 + 0.004
+    k = i - half
+    return 0.5 + 0.5 * math.cos(math.pi * k / half), -0.004
+
+
+def _prop_mesh(radius, hub_radius, hub_height, blades, chord_at, angle_at, thick, nr: int = 24, nc: int = 16) -> TriMesh:
+    m = TriMesh()
+    ns = 24
+    h0, h1 = -hub_height / 2.0, hub_height / 2.0
+    base = 0
+    for k in range(ns):
+        a = 2.0 * math.pi * k / ns
+        c, s = math.cos(a) * hub_radius, math.sin(a) * hub_radius
+        m.positions.append([c, s, h0])
+        m.positions.append([c, s, h1])
+    cb = len(m.positions)
+    m.positions.append([0.0, 0.0, h0])
+    m.positions.append([0.0, 0.0, h1])
+    for k in range(ns):
+        a0 = base + 2 * k
+        a1 = base + 2 * ((k + 1) % ns)
+        m.triangles += [[a0, a1, a1 + 1], [a0, a1 + 1, a0 + 1], [cb, a1, a0], [cb + 1, a0 + 1, a1 + 1]]
+    for b in range(blades):
+        phase = 2.0 * math.pi * b / blades
+        first = len(m.positions)
+        r0 = hub_radius * 0.8
+        for j in range(nr + 1):
+            tt = j / nr
+            r = radius * 0.999 if j == nr else r0 + (radius - r0) * tt
+            c = chord_at(r)
+            beta = angle_at(r)
+            for i in range(nc):
+                s, t = _section_point(i, nc, thick)
+                sc = (0.35 - s) * c
+                tc = t * c
+                along = sc * math.cos(beta) - tc * math.sin(beta)
+                up = sc * math.sin(beta) + tc * math.cos(beta)
+                phi = phase + along / r
+                m.positions.append([r * math.cos(phi), r * math.sin(phi), up])
+        for j in range(nr):
+            for i in range(nc):
+                a = first + j * nc + i
+                b2 = first + j * nc + (i + 1) % nc
+                m.triangles += [[a, a + nc, b2 + nc], [a, b2 + nc, b2]]
+        for ring, flip in ((0, True), (nr, False)):
+            r0i = first + ring * nc
+            for i in range(1, nc - 1):
+                m.triangles.append([r0i, r0i + i, r0i + i + 1] if flip else [r0i, r0i + i + 1, r0i + i])
+    fix_orientation(m)
+    return m
+
+
+def tapered_prop(radius: float = 120.0, chord_root: float = 30.0, chord_tip: float = 12.0, twist_root: float = 35.0,
+                 twist_tip: float = 10.0, thick: float = 0.08, blades: int = 2, hub_radius: float = None,
+                 hub_height: float = None, seed: int = 20261203) -> TriMesh:
+    """A prop in lattice cells about the z axis: chord tapering linearly
+    from chord_root at the hub to chord_tip at the tip, twist (section angle
+    to the rotation plane) from twist_root to twist_tip degrees, thickness a
+    fraction of the chord. `seed` jitters chord (2 %) and twist (1 degree)
+    along the span so the blades are not exactly alike."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    hub_radius = hub_radius if hub_radius is not None else 0.12 * radius
+    hub_height = hub_height if hub_height is not None else 0.9 * chord_root
+    jc = rng.uniform(-0.02, 0.02, 4)
+    jt = rng.uniform(-1.0, 1.0, 4)
+
+    def frac(r):
+        return min(max((r - hub_radius) / (radius - hub_radius), 0.0), 1.0)
+
+    def wobble(j, x):
+        return j[0] + j[1] * math.sin(math.pi * x) + j[2] * math.sin(2 * math.pi * x) + j[3] * math.sin(3 * math.pi * x)
+
+    def chord_at(r):
+        x = frac(r)
+        return (chord_root + (chord_tip - chord_root) * x) * (1.0 + wobble(jc, x) / 4.0)
+
+    def angle_at(r):
+        x = frac(r)
+        return math.radians(twist_root + (twist_tip - twist_root) * x + wobble(jt, x) / 4.0)
+
+    return _prop_mesh(radius, hub_radius, hub_height, blades, chord_at, angle_at, thick)

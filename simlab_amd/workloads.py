@@ -159,3 +159,89 @@ def drone_arm(dims=(512, 256, 256), u=0.06, cs=0.16, re=2000.0, storage: Storage
                          nu=nu, smagorinsky=cs, u_inf=(u, 0.0, 0.0), u_init=(u, 0.0, 0.0), storage=storage,
                          center=(0.3 * nx, ny / 2.0, nz / 2.0), debug_dump=debug_dump)
     return setup, static_geometry(mesh, dims), dict(nu=nu, tau0=0.5 + 3.0 * nu, scale=s, u=u, re=re, seed=seed)
+
+
+def rotation(a, t: float):
+    """tunnel.rs:484-493: rotation matrix about unit axis `a` by angle t
+    (rows of the matrix; its columns are the rotated basis)."""
+    c, s = math.cos(t), math.sin(t)
+    k = 1.0 - c
+    return ((c + a[0] * a[0] * k, a[0] * a[1] * k - a[2] * s, a[0] * a[2] * k + a[1] * s),
+            (a[1] * a[0] * k + a[2] * s, c + a[1] * a[1] * k, a[1] * a[2] * k - a[0] * s),
+            (a[2] * a[0] * k - a[1] * s, a[2] * a[1] * k + a[0] * s, c + a[2] * a[2] * k))
+
+
+class SpinDriver:
+    """Per-step inputs of a spinning part (tunnel.rs:1067-1100): the angular
+    speed ramps in with a smoothstep over `ramp_revs` revolutions, the angle
+    accumulates f * |omega| per step, each step carries its rotation, the
+    previous step's rotation and omega scaled by f."""
+
+    def __init__(self, axis, omega: float, ramp_revs: float = 0.5):
+        n = math.sqrt(sum(a * a for a in axis))
+        self.axis = tuple(a / n for a in axis)
+        self.omega = float(omega)
+        self.ramp = int(math.ceil(ramp_revs * 2.0 * math.pi / abs(omega))) if ramp_revs > 0 else 0
+        self.step = 0
+        self.theta = 0.0
+        self.rot_prev = rotation(self.axis, 0.0)
+
+    def next(self, n: int):
+        from . import StepIn
+        out = []
+        for _ in range(int(n)):
+            s = self.step
+            if s < self.ramp:
+                x = (s + 0.5) / self.ramp
+                f = x * x * (3.0 - 2.0 * x)
+            else:
+                f = 1.0
+            self.theta += f * self.omega
+            rot = rotation(self.axis, self.theta)
+            out.append(StepIn(rot, self.rot_prev, tuple(f * self.omega * a for a in self.axis), False))
+            self.rot_prev = rot
+            self.step += 1
+        return out
+
+
+def spin_box(center, vox, axis_k: int, dims, pad: int = 3):
+    """The swept box of a part spinning about lattice axis `axis_k` through
+    `center` (tunnel.rs:709-721): radius + pad across, the axial extent +
+    pad along, kept one cell inside the domain."""
+    lo, hi = [0.0] * 3, [0.0] * 3
+    for k in range(3):
+        if k == axis_k:
+            lo[k], hi[k] = vox.axial[0], vox.axial[1]
+        else:
+            lo[k], hi[k] = -vox.radius, vox.radius
+    bl, bn = [0] * 3, [0] * 3
+    for k in range(3):
+        a = int(math.floor(center[k] + lo[k] - pad))
+        b = int(math.ceil(center[k] + hi[k] + pad))
+        bl[k] = max(a, 1)
+        bn[k] = max(min(b, dims[k] - 2) - bl[k] + 1, 1)
+    return tuple(bl), tuple(bn)
+
+
+def spinning_prop(dims=(384, 384, 384), radius: float = 120.0, tip: float = 0.08, u_axial: float = 0.01,
+                  cs: float = 0.16, nu: float = 0.001, occ_h: float = 0.125, storage: Storage = Storage.FP16,
+                  seed: int = 20261203, center=None, debug_dump=False):
+    """The spinning-prop tunnel: a seeded two-blade prop (chord 30 -> 12,
+    twist 35 -> 10 degrees, thickness 8 % of the chord, all scaled with
+    radius / 120) spinning about +z at omega = tip / radius rad/step, axial
+    inflow u_axial through a velocity face at -z, open faces elsewhere;
+    body-frame occupancy at spacing occ_h (voxel.rs:509-553)."""
+    from . import voxelize_spin
+    from .shapes import tapered_prop
+    s = radius / 120.0
+    mesh = tapered_prop(radius, 30.0 * s, 12.0 * s, 35.0, 10.0, 0.08, seed=seed)
+    vox = voxelize_spin(mesh.positions, mesh.triangles, (0.0, 0.0, 1.0), occ_h)
+    c = tuple(d / 2.0 for d in dims) if center is None else tuple(center)
+    box_lo, box_n = spin_box(c, vox, 2, dims)
+    setup = LatticeSetup(dims=dims, faces=(FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN, FaceBc.VELOCITY, FaceBc.OPEN),
+                         nu=nu, smagorinsky=cs, u_inf=(0.0, 0.0, u_axial), u_init=(0.0, 0.0, u_axial), storage=storage,
+                         center=c, debug_dump=debug_dump)
+    omega = tip / radius
+    meta = dict(nu=nu, tau0=0.5 + 3.0 * nu, omega=omega, radius=radius, tip=tip, u_axial=u_axial, seed=seed,
+                ntri=int(len(vox.tris)), box_lo=box_lo, box_n=box_n, occ_h=occ_h, axis=(0.0, 0.0, 1.0))
+    return setup, Geometry.spin(vox, box_lo, box_n), meta
