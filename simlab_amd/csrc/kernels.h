@@ -16,10 +16,28 @@ constexpr uint32_t SPECIAL_BLOCK = 256;
 // handled without a record; bits 2-19 then hold its link mask (direction i
 // at bit i + 1). Otherwise bits 2-30 hold the record index + 1.
 constexpr uint32_t FLAG_HALFWAY = 1u << 31;
+// dilation of the near-part grid of a spinning part, in lattice spacings
+// (a link hit needs a triangle within 1.415 * sqrt(3) = 2.45 cells, a solid
+// neighbour or a refill the body within sqrt(2)); a part moves less than a
+// cell per step at any stable tip speed
+constexpr int NEAR_CELLS = 5;
 
-// Device view of one lattice. Population slot of direction i and cell n is
-// element i * ncell + n of `pops` (SoA, x fastest, as shader.rs:147-152).
-// Wall records are SoA too: field j of record r is element j * nrec + r.
+// Populations: cells (index n = (z * ny + y) * nx + x) in groups of
+// POP_GROUP consecutive indices; a group keeps its 19 directions one after
+// the other, so slot (i, n) is element pop_index(i, n) of `pops`. The
+// reference's order (shader.rs:147-152, slot i * ncell + n) is what the
+// population read/write entry points exchange with the caller.
+constexpr uint32_t POP_GROUP = 64;
+__host__ __device__ inline size_t pop_index(uint32_t i, uint32_t n) {
+    return (size_t)(n / POP_GROUP) * (19 * POP_GROUP) + (size_t)i * POP_GROUP + (n % POP_GROUP);
+}
+// elements of the population buffer of a lattice of ncell cells
+__host__ __device__ inline size_t pop_elems(uint64_t ncell) {
+    return (size_t)19 * ((ncell + POP_GROUP - 1) / POP_GROUP) * POP_GROUP;
+}
+
+// Device view of one lattice. Wall records are SoA: field j of record r is
+// element j * nrec + r.
 struct DevLattice {
     uint32_t nx, ny, nz;
     uint32_t ncell;
@@ -70,6 +88,17 @@ struct DevLattice {
     uint32_t ntri;
     const float4 *tris;          // ntri x 3 (v0, e1, e2)
     uint32_t *qscr;              // [nrec * 9 * 3]
+    // body-frame cells of one lattice spacing (from near_lo), two bit planes
+    // of near_words words each (x fastest): plane 0 marks the cells within
+    // NEAR_CELLS of the part's surface or inside it (a box cell outside them
+    // has no link, hit or refill this step), plane 1 those within
+    // NEAR_CELLS + 1 (box cells outside it are not within NEAR_CELLS at the
+    // next step either, so their records need not be kept: they step as
+    // plain fluid cells in the bulk kernel)
+    float near_lo[3];
+    int32_t near_n[3];
+    uint32_t near_words;
+    const uint32_t *near;
 
     // probes
     uint32_t nvert;
@@ -113,7 +142,7 @@ hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 // re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)
 hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
-// resets the link scratch of a spinning part (setup)
+// resets all of the link scratch of a spinning part (setup)
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
 hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 // force history of the first nsteps steps of a chunk from the partials

@@ -2,8 +2,12 @@
 //
 // Semantics follow the reference's WGSL (crates/cfd/src/shader.rs); the
 // data layout and launch structure are this library's own:
-//  * populations: SoA, slot (i, n) = element i * ncell + n, stored as
-//    deviations f~ = f - w_i (FP32 / FP64 / scaled FP16 / BF16);
+//  * populations: cells in groups of 64 consecutive indices, each group
+//    holding its 19 directions one after the other (slot (i, n) = element
+//    (n / 64) * 19 * 64 + i * 64 + n % 64), stored as deviations
+//    f~ = f - w_i (FP32 / FP64 / scaled FP16 / BF16). A wave's 19 loads
+//    of one cell group fall in one contiguous span (2.4 KB FP16), which
+//    HBM serves far better than 19 (parity 1: 38) separate streams;
 //  * AA pattern in place (shader.rs:5-13): a step of parity 0 reads (n, i)
 //    and writes (n, opp i); parity 1 reads (n - c_i, opp i) and writes
 //    (n + c_i, i). Every slot is touched by one thread per step;
@@ -129,8 +133,9 @@ struct Dither {
     }
 };
 
-template <class C> __device__ __forceinline__ typename C::E *popdir(const DevLattice &L, int i) {
-    return (typename C::E *)L.pops + (size_t)i * L.ncell;
+// population slot of direction i of cell n
+template <class C> __device__ __forceinline__ typename C::E &pop(const DevLattice &L, int i, uint32_t n) {
+    return ((typename C::E *)L.pops)[pop_index((uint32_t)i, n)];
 }
 
 // ---------------------------------------------------------------- neighbours
@@ -178,9 +183,9 @@ template <class C, int Q> __device__ __forceinline__ void gather_pre(const DevLa
     sfor<19>([&](auto I) {
         constexpr int i = I;
         if constexpr (Q == 0 || i == 0) {
-            f[i] = dec<C>(popdir<C>(L, i)[nb.n]);
+            f[i] = dec<C>(pop<C>(L, i, nb.n));
         } else {
-            f[i] = dec<C>(popdir<C>(L, OPP[i])[nb.template at<-CX[i], -CY[i], -CZ[i]>()]);
+            f[i] = dec<C>(pop<C>(L, OPP[i], nb.template at<-CX[i], -CY[i], -CZ[i]>()));
         }
     });
 }
@@ -189,9 +194,9 @@ template <class C, int Q> __device__ __forceinline__ void gather_post(const DevL
     sfor<19>([&](auto I) {
         constexpr int i = I;
         if constexpr (Q == 0 || i == 0) {
-            f[i] = dec<C>(popdir<C>(L, OPP[i])[nb.n]);
+            f[i] = dec<C>(pop<C>(L, OPP[i], nb.n));
         } else {
-            f[i] = dec<C>(popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()]);
+            f[i] = dec<C>(pop<C>(L, i, nb.template at<CX[i], CY[i], CZ[i]>()));
         }
     });
 }
@@ -218,13 +223,13 @@ __device__ __forceinline__ void scatter_post(const DevLattice &L, const Nbr &nb,
         if constexpr (Q == 0 || i == 0) {
             constexpr int j = OPP[i];   // slot direction
             const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<j>() : 0u);
-            popdir<C>(L, j)[nb.n] = e;
+            pop<C>(L, j, nb.n) = e;
             if constexpr (RT) f[i] = dec<C>(e);
         } else {
             bool ok = true;
             if constexpr (GUARD) ok = keeps_inside<i>(L, nb);
             const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
-            if (ok) popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()] = e;
+            if (ok) pop<C>(L, i, nb.template at<CX[i], CY[i], CZ[i]>()) = e;
             if constexpr (RT) f[i] = dec<C>(e);
         }
     });
@@ -235,7 +240,7 @@ template <class C> __device__ __forceinline__ void scatter_pre0(const DevLattice
     if constexpr (C::SR) d.init(n, seed);
     sfor<19>([&](auto I) {
         constexpr int i = I;
-        popdir<C>(L, i)[n] = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
+        pop<C>(L, i, n) = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
     });
 }
 
@@ -642,7 +647,7 @@ __device__ __forceinline__ void face_store(const DevLattice &L, int bx, int by, 
         dir = (uint32_t)i;
     }
     const uint32_t d16 = C::SR ? (cell_hash(b * 19u + dir, seed) >> 16) : 0u;
-    ((typename C::E *)L.pops + (size_t)dir * L.ncell)[cell] = enc<C>(v, d16);
+    pop<C>(L, (int)dir, cell) = enc<C>(v, d16);
 }
 
 // Thread-serial face update of one face cell (standalone kernel)
@@ -659,6 +664,11 @@ __device__ __forceinline__ void face_cell_write(const DevLattice &L, int bx, int
 __device__ __forceinline__ bool is_face_source(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z) {
     return (!L.per[0] && (x == 1 || x == L.nx - 2)) || (!L.per[1] && (y == 1 || y == L.ny - 2)) ||
            (!L.per[2] && (z == 1 || z == L.nz - 2));
+}
+// the face cells: every cell on a face of a non-periodic axis (lattice.rs:283-324)
+__device__ __forceinline__ bool is_face_cell(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z) {
+    return (!L.per[0] && (x == 0 || x == L.nx - 1)) || (!L.per[1] && (y == 0 || y == L.ny - 1)) ||
+           (!L.per[2] && (z == 0 || z == L.nz - 1));
 }
 
 // Wave-cooperative face update after a step of parity Q: for every lane
@@ -788,20 +798,33 @@ __device__ __forceinline__ HalfwayLink<T> halfway_link(const DevLattice &L, cons
 // f_post,i of the previous step, read from where it was stored
 template <class C, int P, int i>
 __device__ __forceinline__ typename C::T halfway_fpi(const DevLattice &L, const Nbr &nb) {
-    if constexpr (P == 0) return dec<C>(popdir<C>(L, i)[nb.template at<CX[i], CY[i], CZ[i]>()]);
-    else return dec<C>(popdir<C>(L, OPP[i])[nb.n]);
+    if constexpr (P == 0) return dec<C>(pop<C>(L, i, nb.template at<CX[i], CY[i], CZ[i]>()));
+    else return dec<C>(pop<C>(L, OPP[i], nb.n));
+}
+
+// All 18 slots are loaded, linked or not (each is inside the buffer: Nbr
+// wraps at the domain ends), so the loads go out together instead of one
+// round trip per link; the unlinked ones are dropped.
+template <class C, int P>
+__device__ __forceinline__ void halfway_fp(const DevLattice &L, const Nbr &nb, typename C::T fp[19]) {
+    sfor<18>([&](auto I) {
+        constexpr int i = I + 1;
+        fp[i] = halfway_fpi<C, P, i>(L, nb);
+    });
 }
 
 template <class C, int P>
 __device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
                                               typename C::T f[19]) {
     using T = typename C::T;
+    T fp[19];
+    halfway_fp<C, P>(L, nb, fp);
     sfor<18>([&](auto I) {
         constexpr int i = I + 1;
         if (m & (1u << i)) {
             const T wi = wgt<T>(i);
             const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
-            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            const T fpi = fp[i] + wi;
             f[OPP[i]] = (fpi - (T)6 * wi * h.cu) - wi;
         }
     });
@@ -813,12 +836,14 @@ template <class C, int P>
 __device__ __forceinline__ void halfway_force(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
                                               typename C::T F[3], typename C::T Tq[3]) {
     using T = typename C::T;
+    T fp[19];
+    halfway_fp<C, P>(L, nb, fp);
     sfor<18>([&](auto I) {
         constexpr int i = I + 1;
         if (m & (1u << i)) {
             const T wi = wgt<T>(i);
             const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
-            const T fpi = halfway_fpi<C, P, i>(L, nb) + wi;
+            const T fpi = fp[i] + wi;
             const T fb = fpi - (T)6 * wi * h.cu;
             const T flx = (h.cx - h.uwx) * fpi + (h.cx + h.uwx) * fb;
             const T fly = (h.cy - h.uwy) * fpi + (h.cy + h.uwy) * fb;
@@ -1043,15 +1068,21 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     const bool inside = x < L.nx && y < L.ny && ty < L.by;
     const uint32_t n = (z * L.ny + y) * L.nx + x;
     const uint32_t fg = inside ? L.flags[n] : 1u;
+    // The populations are loaded without waiting for the flag, so the two
+    // round trips overlap: every cell that may collide (all but the face
+    // cells, known from the coordinates) loads, every index being inside the
+    // buffer (Nbr wraps at the domain ends); solid and wall-record cells
+    // among them drop what they loaded.
+    const bool may = inside && !is_face_cell(L, x, y, z);
+    Nbr nb;
+    nb.make(L, x, y, z, n);
+    T f[19];
+    if (may) gather_pre<C, P>(L, nb, f);
     // ours: fluid cells without a record (plain or halfway wall cells);
     // solid, face and wall-record cells are not
-    const bool mine = (fg & 3u) == 0 && rec_of(fg) == 0;
+    const bool mine = may && (fg & 3u) == 0 && rec_of(fg) == 0;
     const uint32_t hwm = (fg & FLAG_HALFWAY) ? ((fg >> 1) & 0x7FFFEu) : 0u;
-    T f[19];
     if (mine) {
-        Nbr nb;
-        nb.make(L, x, y, z, n);
-        gather_pre<C, P>(L, nb, f);
         if (hwm) halfway_links<C, P>(L, a, nb, hwm, f);
         T rho, u[3], pp[6];
         collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
@@ -1151,6 +1182,26 @@ __device__ __forceinline__ bool solid_at(const DevLattice &L, const StepArgs &a,
     const V3f pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
     return occ_at(L, to_body(before ? a.q : a.r, pc));
 }
+// the cell's body-frame position lies in the marked cells of the near-part
+// grid's plane 0 (within NEAR_CELLS) or plane 1 (within NEAR_CELLS + 1)
+__device__ __forceinline__ bool near_part(const DevLattice &L, V3f b, int plane = 0) {
+    const int gx = (int)__builtin_floorf(b.x - L.near_lo[0]);
+    const int gy = (int)__builtin_floorf(b.y - L.near_lo[1]);
+    const int gz = (int)__builtin_floorf(b.z - L.near_lo[2]);
+    if (gx < 0 || gy < 0 || gz < 0 || gx >= L.near_n[0] || gy >= L.near_n[1] || gz >= L.near_n[2]) return false;
+    const uint32_t l = (uint32_t)((gz * L.near_n[1] + gy) * L.near_n[0] + gx);
+    return ((L.near[plane * L.near_words + (l >> 5)] >> (l & 31u)) & 1u) != 0u;
+}
+// box cell k -> lattice coordinates and position relative to the centre
+__device__ __forceinline__ void box_cell(const DevLattice &L, uint32_t k, int &px, int &py, int &pz, V3f &pc) {
+    const uint32_t BX = L.box_n[0], BY = L.box_n[1];
+    px = (int)(k % BX) + L.box_lo[0];
+    py = (int)((k / BX) % BY) + L.box_lo[1];
+    pz = (int)(k / (BX * BY)) + L.box_lo[2];
+    const V3f c = centerf(L);
+    pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
+}
+
 // Moller-Trumbore for the segment o + t d, t in [0, 1]; 9 for no hit
 __device__ __forceinline__ float ray_tri(V3f o, V3f d, V3f v0, V3f e1, V3f e2) {
 #pragma clang fp contract(off)
@@ -1171,9 +1222,11 @@ __device__ __forceinline__ float ray_tri(V3f o, V3f d, V3f v0, V3f e1, V3f e2) {
     return t > 0.0f ? t : 0.0f;
 }
 
-__global__ void __launch_bounds__(64) k_tri_scatter(DevLattice L, StepArgs a) {
+// one wave per triangle; its lanes share the lattice cells of the
+// triangle's bounding box (grown by 1.415)
+__global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
 #pragma clang fp contract(off)
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
     if (t >= L.ntri) return;
     const float4 t0 = L.tris[3 * t], t1 = L.tris[3 * t + 1], t2 = L.tris[3 * t + 2];
     const V3f A = centerf(L) + from_body(a.r, v3(t0.x, t0.y, t0.z));
@@ -1194,28 +1247,29 @@ __global__ void __launch_bounds__(64) k_tri_scatter(DevLattice L, StepArgs a) {
             hi[k] = min((int)__builtin_floorf(mx + 1.415f), L.box_lo[k] + (int)L.box_n[k] - 1);
         }
     }
-    for (int z = lo[2]; z <= hi[2]; ++z)
-        for (int y = lo[1]; y <= hi[1]; ++y)
-            for (int x = lo[0]; x <= hi[0]; ++x) {
-                const V3f p = v3((float)x, (float)y, (float)z);
-                const float d0 = dot3(nn, p - A);
-                if (__builtin_fabsf(d0) > 1.415f) continue;
-                const uint32_t k = ((uint32_t)(z - L.box_lo[2]) * L.box_n[1] + (uint32_t)(y - L.box_lo[1])) * L.box_n[0] +
-                                   (uint32_t)(x - L.box_lo[0]);
-                for (int i = 1; i < 19; i += 2) {
-                    const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
-                    const float d1 = d0 + dot3(nn, ci);
-                    if (d0 * d1 > 0.0f && __builtin_fabsf(d0) > 1e-6f) continue;
-                    const float tt = ray_tri(p, ci, A, e1, e2);
-                    if (tt <= 1.0f) {
-                        uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
-                        const uint32_t tb = __float_as_uint(tt);   // tt >= +0 (ray_tri)
-                        atomicMin(o, tb);
-                        atomicMax(o + 1, tb);
-                        atomicAdd(o + 2, 1u);
-                    }
-                }
+    if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return;
+    const uint32_t ex = (uint32_t)(hi[0] - lo[0] + 1), ey = (uint32_t)(hi[1] - lo[1] + 1), ez = (uint32_t)(hi[2] - lo[2] + 1);
+    for (uint32_t c = lane; c < ex * ey * ez; c += 64) {
+        const int x = lo[0] + (int)(c % ex), y = lo[1] + (int)((c / ex) % ey), z = lo[2] + (int)(c / (ex * ey));
+        const V3f p = v3((float)x, (float)y, (float)z);
+        const float d0 = dot3(nn, p - A);
+        if (__builtin_fabsf(d0) > 1.415f) continue;
+        const uint32_t k = ((uint32_t)(z - L.box_lo[2]) * L.box_n[1] + (uint32_t)(y - L.box_lo[1])) * L.box_n[0] +
+                           (uint32_t)(x - L.box_lo[0]);
+        for (int i = 1; i < 19; i += 2) {
+            const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
+            const float d1 = d0 + dot3(nn, ci);
+            if (d0 * d1 > 0.0f && __builtin_fabsf(d0) > 1e-6f) continue;
+            const float tt = ray_tri(p, ci, A, e1, e2);
+            if (tt <= 1.0f) {
+                uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
+                const uint32_t tb = __float_as_uint(tt);   // tt >= +0 (ray_tri)
+                atomicMin(o, tb);
+                atomicMax(o + 1, tb);
+                atomicAdd(o + 2, 1u);
             }
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_qclear(DevLattice L) {
@@ -1224,6 +1278,23 @@ __global__ void __launch_bounds__(256) k_qclear(DevLattice L) {
     L.qscr[3 * (size_t)k] = 0x7f7fffffu;
     L.qscr[3 * (size_t)k + 1] = 0u;
     L.qscr[3 * (size_t)k + 2] = 0u;
+}
+// after a step: only cells near the part can hold hits of this step's
+// triangles (every other entry is still clear from the step before)
+__global__ void __launch_bounds__(256) k_qclear_near(DevLattice L, StepArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L.nrec) return;
+    int px, py, pz;
+    V3f pc;
+    box_cell(L, k, px, py, pz, pc);
+    if (!near_part(L, to_body(a.r, pc))) return;
+    uint32_t *o = L.qscr + (size_t)k * 27;
+#pragma unroll
+    for (int w = 0; w < 9; ++w) {
+        o[3 * w] = 0x7f7fffffu;
+        o[3 * w + 1] = 0u;
+        o[3 * w + 2] = 0u;
+    }
 }
 
 // f16 round to nearest even of a q in [0, 1] (pack2x16float)
@@ -1247,7 +1318,7 @@ __device__ __forceinline__ void refill_store(const DevLattice &L, const Nbr &nb,
             dir = i;
         }
         const uint32_t d16 = C::SR ? (cell_hash(nb.n * 19u + dir, seed) >> 16) : 0u;
-        popdir<C>(L, dir)[cell] = enc<C>(fe[i], d16);
+        pop<C>(L, dir, cell) = enc<C>(fe[i], d16);
     });
 }
 
@@ -1261,12 +1332,26 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
     const uint32_t n = ((uint32_t)pz * L.ny + (uint32_t)py) * L.nx + (uint32_t)px;
     const V3f c = centerf(L);
     const V3f pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
-    const bool solid_now = solid_at(L, a, px, py, pz, false);
-    const bool solid_before = solid_at(L, a, px, py, pz, true);
     const size_t NR = L.nrec;
     T *rfpost = (T *)L.rfpost;
     T *rmac = (T *)L.rmac;
     T *rft = (T *)L.rft;
+    const V3f bnow = to_body(a.r, pc);
+    if (!near_part(L, bnow)) {
+        // far from the part: fluid now and before, no solid neighbour, no
+        // hit; the outcome of the full rule below is known. Farther still
+        // (outside plane 1) the cell steps as a plain fluid cell in the
+        // bulk kernel and its record is left as it is.
+#pragma unroll
+        for (int w = 0; w < 9; ++w) L.rq[(size_t)w * NR + k] = 0x3C003C00u;
+        L.rmask[k] = 0u;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) rft[j * NR + k] = (T)0;
+        L.flags[n] = near_part(L, bnow, 1) ? (k + 1u) << 2 : 0u;
+        return;
+    }
+    const bool solid_now = solid_at(L, a, px, py, pz, false);
+    const bool solid_before = solid_at(L, a, px, py, pz, true);
     const T pcx = (T)pc.x, pcy = (T)pc.y, pcz = (T)pc.z;
     if (solid_now) {
         L.flags[n] = 1u | ((k + 1u) << 2);
@@ -1616,7 +1701,7 @@ __global__ void __launch_bounds__(256) k_adj_faces(DevLattice L, DevLattice A, c
                         cell = ((uint32_t)tz * L.ny + (uint32_t)tyy) * L.nx + (uint32_t)tx;
                         dir = (uint32_t)i;
                     }
-                    T *slot = adj + (size_t)dir * L.ncell + cell;
+                    T *slot = adj + pop_index(dir, cell);
                     const T ab = *slot;
                     *slot = (T)0;
                     if (ab == (T)0) continue;
@@ -1653,8 +1738,8 @@ __global__ void __launch_bounds__(256) k_adj_faces(DevLattice L, DevLattice A, c
     sfor<19>([&](auto I) {
         constexpr int i = I;
         T *slot;
-        if constexpr (P == 0 || i == 0) slot = adj + (size_t)OPP[i] * L.ncell + nb.n;
-        else slot = adj + (size_t)i * L.ncell + nb.template at<CX[i], CY[i], CZ[i]>();
+        if constexpr (P == 0 || i == 0) slot = adj + pop_index(OPP[i], nb.n);
+        else slot = adj + pop_index(i, nb.template at<CX[i], CY[i], CZ[i]>());
         *slot = *slot + fb[i];
     });
 }
@@ -1710,8 +1795,8 @@ __global__ void __launch_bounds__(1024) k_adj_step(DevLattice L, DevLattice A, c
     sfor<19>([&](auto I) {
         constexpr int i = I;
         const T v = fb[i] + ain[i];
-        if constexpr (P == 0 || i == 0) adj[(size_t)i * L.ncell + nb.n] = v;
-        else adj[(size_t)OPP[i] * L.ncell + nb.template at<-CX[i], -CY[i], -CZ[i]>()] = v;
+        if constexpr (P == 0 || i == 0) adj[pop_index(i, nb.n)] = v;
+        else adj[pop_index(OPP[i], nb.template at<-CX[i], -CY[i], -CZ[i]>())] = v;
     });
 }
 
@@ -2056,7 +2141,7 @@ hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hip
 }
 
 hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 64), 64, 0, s>>>(L, a);
+    if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 4), 256, 0, s>>>(L, a);
     const uint32_t nb = cdiv(L.nrec, 256);
 #define VOX(C)                                                          \
     do {                                                                \
@@ -2070,7 +2155,7 @@ hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_
         default: VOX(CF64); break;
     }
 #undef VOX
-    k_qclear<<<cdiv((uint64_t)L.nrec * 9, 256), 256, 0, s>>>(L);
+    k_qclear_near<<<nb, 256, 0, s>>>(L, a);
     return hipGetLastError();
 }
 

@@ -119,7 +119,7 @@ struct simlab_lattice {
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
     DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
-        counter, dump, mass_part, mass_out, occ, tris, qscr, verts, vacc, slicedef, sacc;
+        counter, dump, mass_part, mass_out, occ, tris, qscr, near, verts, vacc, slicedef, sacc;
     bool fused_faces = true;   // face update fused into the previous step
     // partial bounce-back and the drag adjoint
     DevBuf phi, grad, adj, ckpt, seg, pbb_part;
@@ -169,9 +169,18 @@ const char *validate(const simlab_lattice_setup *s, std::string &msg) {
 
 // Launch shape: whole x-rows per block. bx = nx rounded up to a wave (at most
 // 1024 lanes), by rows so a block has about `target` threads.
+// Default tile: 16-bit storages run best with short tiles (128 x 1: two
+// waves per block, so more blocks share a CU), 32/64-bit ones with whole
+// rows (measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
+// searches per lattice).
 void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) {
     uint32_t bx = ((L.nx + 63) / 64) * 64;
     if (bx > 1024) bx = 1024;
+    const bool half = L.storage == SIMLAB_STORAGE_FP16 || L.storage == SIMLAB_STORAGE_BF16;
+    if (half && bx > 128 && !std::getenv("SIMLAB_CFD_BLOCK")) {
+        bx = 128;
+        target_threads = 128;
+    }
     uint32_t by = by_override ? by_override : std::max<uint32_t>(1, target_threads / bx);
     by = std::max<uint32_t>(1, std::min<uint32_t>(by, std::min<uint32_t>(1024 / bx, L.ny)));
     L.bx = bx;
@@ -179,6 +188,89 @@ void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) 
     L.xblocks = (L.nx + bx - 1) / bx;
     L.yblocks = (L.ny + by - 1) / by;
     L.wall_blocks = std::max<uint32_t>(1, (L.nrec + SPECIAL_BLOCK - 1) / SPECIAL_BLOCK);
+}
+
+// between the device's grouped population order and the reference's
+// direction-major order (slot i * ncell + n): to_ref copies dev -> ref,
+// otherwise ref -> dev
+void reorder_pops(uint64_t ncell, size_t eb, char *dev, char *ref, bool to_ref) {
+    for (uint32_t i = 0; i < 19; ++i)
+        for (uint64_t n = 0; n < ncell; ++n) {
+            char *d = dev + pop_index(i, (uint32_t)n) * eb;
+            char *r = ref + ((size_t)i * ncell + n) * eb;
+            if (to_ref) std::memcpy(r, d, eb);
+            else std::memcpy(d, r, eb);
+        }
+}
+
+// Body-frame grid of unit cells from lo: a cell is marked when it holds a
+// solid occupancy sample or meets a triangle's bounding box, then the marks
+// are dilated by NEAR_CELLS cells (Chebyshev; first bit plane) and by one
+// more (second plane). A point in an unmarked cell of the first plane is
+// farther than NEAR_CELLS - 1 from every surface and solid sample.
+std::vector<uint32_t> near_grid(const simlab_geometry &g, float lo[3], int32_t n[3]) {
+    double flo[3];
+    for (int k = 0; k < 3; ++k) {
+        flo[k] = g.occ_lo[k] - NEAR_CELLS - 2.0;
+        n[k] = (int32_t)std::ceil(g.occ_dims[k] * g.occ_h) + 2 * NEAR_CELLS + 5;
+        lo[k] = (float)flo[k];
+    }
+    const size_t nx = n[0], ny = n[1], nz = n[2];
+    std::vector<uint8_t> m(nx * ny * nz, 0);
+    auto cell = [&](double v, int k) {
+        return std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(v - flo[k]), 0), n[k] - 1);
+    };
+    const uint64_t ox = g.occ_dims[0], oy = g.occ_dims[1];
+    for (uint64_t w = 0; w < g.nocc_words; ++w) {
+        uint32_t bits = g.occ_bits[w];
+        while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            const uint64_t l = w * 32 + b;
+            const uint64_t i = l % ox, j = (l / ox) % oy, kk = l / (ox * oy);
+            const double p[3] = {g.occ_lo[0] + i * g.occ_h, g.occ_lo[1] + j * g.occ_h, g.occ_lo[2] + kk * g.occ_h};
+            m[((size_t)cell(p[2], 2) * ny + cell(p[1], 1)) * nx + cell(p[0], 0)] = 1;
+        }
+    }
+    for (uint64_t t = 0; t < g.ntri; ++t) {
+        const float *r = g.tris + 12 * t;
+        int64_t a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            const double v0 = r[k], v1 = r[k] + r[4 + k], v2 = r[k] + r[8 + k];
+            a[k] = cell(std::min(std::min(v0, v1), v2), k);
+            b[k] = cell(std::max(std::max(v0, v1), v2), k);
+        }
+        for (int64_t z = a[2]; z <= b[2]; ++z)
+            for (int64_t y = a[1]; y <= b[1]; ++y)
+                for (int64_t x = a[0]; x <= b[0]; ++x) m[((size_t)z * ny + y) * nx + x] = 1;
+    }
+    // separable max filter of radius r along x, y, z
+    std::vector<uint8_t> t2(m.size());
+    const size_t stride[3] = {1, nx, nx * ny};
+    auto dilate = [&](int r) {
+        for (int ax = 0; ax < 3; ++ax) {
+            const int64_t len = n[ax];
+            for (size_t c = 0; c < m.size(); ++c) {
+                const int64_t pos = (int64_t)((c / stride[ax]) % (size_t)len);
+                uint8_t v = 0;
+                for (int64_t d = -r; d <= r && !v; ++d) {
+                    const int64_t q = pos + d;
+                    if (q >= 0 && q < len) v = m[c + (size_t)(q - pos) * stride[ax]];
+                }
+                t2[c] = v;
+            }
+            m.swap(t2);
+        }
+    };
+    const size_t words = (m.size() + 31) / 32;
+    std::vector<uint32_t> bits(2 * words, 0);
+    dilate(NEAR_CELLS);
+    for (size_t c = 0; c < m.size(); ++c)
+        if (m[c]) bits[c >> 5] |= 1u << (c & 31);
+    dilate(1);
+    for (size_t c = 0; c < m.size(); ++c)
+        if (m[c]) bits[words + (c >> 5)] |= 1u << (c & 31);
+    return bits;
 }
 
 // an explicit tile of bx x-cells (a multiple of 64) by `by` rows
@@ -397,7 +489,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     size_t mem = 0;
     const size_t eb = elem_bytes(su->storage);
     try {
-        lat->pops.alloc((size_t)19 * n * eb, mem);
+        lat->pops.alloc(pop_elems(n) * eb, mem);
         lat->flags.alloc(n * 4, mem);
         lat->rmask.alloc((size_t)nrec * 4, mem);
         lat->rcell.alloc((size_t)nrec * 4, mem);
@@ -418,7 +510,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     } catch (const HipError &e) {
         if (e.code == SIMLAB_ERR_OOM) {
             char b[160];
-            std::snprintf(b, sizeof b, "not enough GPU memory for this grid (%.1f GB): %s", (double)(19 * n * eb + n * 4) / 1e9, e.what());
+            std::snprintf(b, sizeof b, "not enough GPU memory for this grid (%.1f GB): %s", (double)(pop_elems(n) * eb + n * 4) / 1e9, e.what());
             return fail(SIMLAB_ERR_OOM, b);
         }
         throw;
@@ -503,6 +595,14 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         L.ntri = (uint32_t)g->ntri;
         L.tris = lat->tris.as<float4>();
         L.qscr = lat->qscr.as<uint32_t>();
+        int32_t nn[3];
+        std::vector<uint32_t> near = near_grid(*g, L.near_lo, nn);
+        for (int k = 0; k < 3; ++k) L.near_n[k] = nn[k];
+        L.near_words = (uint32_t)(near.size() / 2);
+        lat->near.alloc(near.size() * 4, mem);
+        hip_check(hipMemcpyAsync(lat->near.p, near.data(), near.size() * 4, hipMemcpyHostToDevice, st), "upload");
+        hip_check(hipStreamSynchronize(st), "GPU wait failed");
+        L.near = lat->near.as<uint32_t>();
     }
     L.forcing = (su->body_force[0] != 0.0 || su->body_force[1] != 0.0 || su->body_force[2] != 0.0) ? 1 : 0;
     L.sponge = 0;
@@ -845,12 +945,15 @@ int simlab_cfd_lattice_mass(simlab_lattice *lat, double *mass) {
 int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes) {
     return guarded([&]() -> int {
         if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
-        const size_t want = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+        const size_t eb = elem_bytes(lat->storage);
+        const size_t want = (size_t)19 * lat->ncell * eb;
         if (bytes != want) return fail(SIMLAB_ERR_INVALID, "population buffer has the wrong size");
         try {
             hip_check(hipSetDevice(lat->device), "hipSetDevice");
-            hip_check(hipMemcpyAsync(out, lat->pops.p, want, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+            std::vector<char> dev(pop_elems(lat->ncell) * eb);
+            hip_check(hipMemcpyAsync(dev.data(), lat->pops.p, dev.size(), hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
             hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            reorder_pops(lat->ncell, eb, dev.data(), (char *)out, true);
             return SIMLAB_OK;
         } catch (const HipError &e) {
             return fail(e.code, e.what());
@@ -861,11 +964,14 @@ int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes) {
 int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t bytes) {
     return guarded([&]() -> int {
         if (!lat || !in) return fail(SIMLAB_ERR_INVALID, "null argument");
-        const size_t want = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+        const size_t eb = elem_bytes(lat->storage);
+        const size_t want = (size_t)19 * lat->ncell * eb;
         if (bytes != want) return fail(SIMLAB_ERR_INVALID, "population buffer has the wrong size");
         try {
             hip_check(hipSetDevice(lat->device), "hipSetDevice");
-            hip_check(hipMemcpyAsync(lat->pops.p, in, want, hipMemcpyHostToDevice, lat->stream), "GPU write failed");
+            std::vector<char> dev(pop_elems(lat->ncell) * eb, 0);
+            reorder_pops(lat->ncell, eb, dev.data(), (char *)const_cast<void *>(in), false);
+            hip_check(hipMemcpyAsync(lat->pops.p, dev.data(), dev.size(), hipMemcpyHostToDevice, lat->stream), "GPU write failed");
             hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
             return SIMLAB_OK;
         } catch (const HipError &e) {
@@ -943,7 +1049,7 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
             }
             // the trial steps advance the state: keep it and put it back
             hipStream_t st = lat->stream;
-            const size_t pb = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            const size_t pb = pop_elems(lat->ncell) * elem_bytes(lat->storage);
             DevBuf save_pops, save_fpost, save_mac, save_ft;
             uint64_t mem = 0;
             save_pops.alloc(pb, mem);
@@ -1180,7 +1286,7 @@ int simlab_cfd_adjoint_forward(simlab_lattice *lat, size_t nsteps, size_t ckpt_e
             hip_check(hipSetDevice(lat->device), "hipSetDevice");
             DevLattice &L = lat->L;
             hipStream_t st = lat->stream;
-            const size_t state = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            const size_t state = pop_elems(lat->ncell) * elem_bytes(lat->storage);
             const size_t nck = (nsteps + ckpt_every - 1) / ckpt_every;
             const uint32_t nblk = L.xblocks * L.yblocks * L.nz;
             uint64_t mem = 0;
@@ -1250,7 +1356,7 @@ int simlab_cfd_adjoint_backward(simlab_lattice *lat, double *dJ_dphi) {
             hip_check(hipSetDevice(lat->device), "hipSetDevice");
             DevLattice &L = lat->L;
             hipStream_t st = lat->stream;
-            const size_t state = (size_t)19 * lat->ncell * elem_bytes(lat->storage);
+            const size_t state = pop_elems(lat->ncell) * elem_bytes(lat->storage);
             const uint64_t T = lat->adj_steps, K = lat->adj_every;
             uint64_t mem = 0;
             try {
