@@ -22,6 +22,8 @@ Reported (one JSON line):
                    storage and its error against the FP64 run
   cpu_baseline     the CPU oracle (C++ restatement of the reference kernels)
                    on a bounded sample of the same workload
+  n384             the 384^3 cases (sphere channel, spinning prop), FP16 and
+                   FP32, with their whole-step roofline fractions
 """
 import argparse
 import json
@@ -60,12 +62,61 @@ def timed_run(sa, W, storage, steps, warmup):
         "nrec": lat.nrec,
     }
     res["bulk_cells"] = lat.bulk_cells
+    res["tile"] = list(lat.tile())
     b = bytes_per_cell(storage) * lat.bulk_cells   # algorithmic bytes of one bulk launch
     res["kernel_gbs"] = b / (main_ms / launches * 1e-3) / 1e9
     res["kernel_roofline_frac"] = res["kernel_gbs"] / HBM_PEAK_GBS
     res["roofline_frac"] = res["mlups"] * 1e6 * bytes_per_cell(storage) / (HBM_PEAK_GBS * 1e9)
     del lat
     return res, n
+
+
+def large_runs(sa, W, steps, warmup):
+    """The 384^3 cases: the sphere channel scaled to 384^3 (sphere r=24 at
+    x=96, Re 1000, Cs 0.1) and the spinning two-blade prop (radius 120,
+    tip speed 0.08, axial inflow 0.01, Cs 0.16), FP16 and FP32 storage;
+    MLUPS over all cells from the wall clock, the bulk kernel's algorithmic
+    GB/s from its HIP events."""
+    out = {}
+    for name, st in (("fp16", sa.Storage.FP16), ("fp32", sa.Storage.FP32)):
+        print(f"# 384^3 sphere channel {name}", file=sys.stderr, flush=True)
+        setup, geom, meta = W.sphere_channel((384, 384, 384), 24.0, 96.0, 1000.0, 0.05, 0.1, st)
+        lat = sa.Lattice(setup, geom)
+        lat.run_still(warmup)
+        t0 = time.perf_counter()
+        lat.run_still(steps)
+        wall = time.perf_counter() - t0
+        lat.set_timing(True)
+        lat.run_still(steps)
+        step_ms, main_ms, launches = lat.timing()
+        gbs = bytes_per_cell(st) * lat.bulk_cells / (main_ms / launches * 1e-3) / 1e9
+        mlups = setup.cells() * steps / wall / 1e6
+        out[f"sphere384_{name}"] = {
+            "mlups": round(mlups, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+            "roofline_frac": round(mlups * 1e6 * bytes_per_cell(st) / (HBM_PEAK_GBS * 1e9), 4),
+            "kernel_gbs": round(gbs, 1), "kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "tile": list(lat.tile())}
+        del lat
+    for name, st in (("fp16", sa.Storage.FP16), ("fp32", sa.Storage.FP32)):
+        print(f"# 384^3 spinning prop {name}", file=sys.stderr, flush=True)
+        setup, geom, meta = W.spinning_prop(storage=st)
+        lat = sa.Lattice(setup, geom)
+        drv = W.SpinDriver(meta["axis"], meta["omega"])
+        lat.run(drv.next(warmup))
+        batch = drv.next(steps)
+        t0 = time.perf_counter()
+        h = lat.run(batch)
+        wall = time.perf_counter() - t0
+        mlups = setup.cells() * steps / wall / 1e6
+        out[f"prop384_{name}"] = {
+            "mlups": round(mlups, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+            "roofline_frac": round(mlups * 1e6 * bytes_per_cell(st) / (HBM_PEAK_GBS * 1e9), 4),
+            "box_cells": int(lat.nrec), "triangles": meta["ntri"],
+            "thrust_z_last": float(h[-1, 2]), "torque_z_last": float(h[-1, 5])}
+        del lat
+    if "sphere384_fp16" in out and "sphere384_fp32" in out:
+        out["sphere384_fp16_vs_fp32"] = round(out["sphere384_fp16"]["mlups"] / out["sphere384_fp32"]["mlups"], 3)
+    return out
 
 
 def drag_run(sa, W, storage, steps):
@@ -137,6 +188,9 @@ def main():
     ap.add_argument("--drag-steps", type=int, default=5000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
+    ap.add_argument("--no-384", action="store_true", help="skip the 384^3 sphere-channel and spinning-prop timings")
+    ap.add_argument("--only-384", action="store_true",
+                    help="only the 384^3 timings (one JSON object of them; for profiling those kernels alone)")
     ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel that scales the "
@@ -156,6 +210,9 @@ def main():
 
     info = sa.gpu_info()
     print(f"# {info}", file=sys.stderr, flush=True)
+    if args.only_384:
+        print(json.dumps({"n384": large_runs(sa, W, args.steps, args.warmup)}), flush=True)
+        return
     storages = [s for s in (args.only.split(",") if args.only else ["fp16", "fp32", "bf16"]) if s]
     results = {}
     n = None
@@ -175,6 +232,10 @@ def main():
         for name in ("fp32", "fp16", "bf16"):
             drag[name]["cd_err_vs_fp64"] = drag[name]["cd"] / drag["fp64"]["cd"] - 1.0
         drag["window"] = f"mean over steps {args.drag_steps // 2}..{args.drag_steps}"
+
+    large = None
+    if not args.no_384:
+        large = large_runs(sa, W, max(20, min(args.steps, 200)), 10)
 
     cpu = None
     if not args.no_cpu:
@@ -215,6 +276,8 @@ def main():
         out["fp16_vs_fp32_speedup"] = round(results["fp16"]["mlups"] / results["fp32"]["mlups"], 3)
     if drag is not None:
         out["drag"] = drag
+    if large is not None:
+        out["n384"] = large
     print(json.dumps(out), flush=True)
 
 

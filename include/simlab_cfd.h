@@ -246,6 +246,8 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
                                 double *step_us_out);
 /* Sets the tile of the stream-collide kernel explicitly. */
 int simlab_cfd_lattice_set_tile(simlab_lattice *lat, uint32_t bx, uint32_t by);
+/* The tile in use: tile_out = {bx, by}. */
+int simlab_cfd_lattice_get_tile(const simlab_lattice *lat, uint32_t *tile_out);
 
 /* Batches (new): several independent lattices (one storage, the same body
  * force and sponge settings, static geometry, the same steps_done) stepped

@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_lattice_timing": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64)]),
         "simlab_cfd_lattice_autotune": (I, [V, C.c_uint32, I, P, C.POINTER(D)]),
         "simlab_cfd_lattice_set_tile": (I, [V, C.c_uint32, C.c_uint32]),
+        "simlab_cfd_lattice_get_tile": (I, [V, V]),
         "simlab_cfd_batch_new": (I, [P, C.c_size_t, C.POINTER(V)]),
         "simlab_cfd_batch_run_still": (I, [V, C.c_size_t, P]),
         "simlab_cfd_batch_free": (None, [V]),
@@ -581,6 +582,12 @@ class Lattice:
 
     def set_tile(self, bx: int, by: int):
         _check(self._lib, self._lib.simlab_cfd_lattice_set_tile(self._h, int(bx), int(by)))
+
+    def tile(self):
+        """(bx, by) of the stream-collide kernel's blocks."""
+        t = np.zeros(2, np.uint32)
+        _check(self._lib, self._lib.simlab_cfd_lattice_get_tile(self._h, _ptr(t)))
+        return int(t[0]), int(t[1])
 
     # ---- partial bounce-back and the drag adjoint (no counterpart in the reference)
     def set_solid_fraction(self, phi):

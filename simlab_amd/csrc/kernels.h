@@ -138,6 +138,8 @@ hipError_t launch_init(const DevLattice &L, hipStream_t s);
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
 // k_special on s_special, k_bulk on s_bulk (independent within a step)
 hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
+// the same as one kernel on one stream (the tile's bx * by must divide SPECIAL_BLOCK)
+hipError_t launch_step_fused(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 // re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)

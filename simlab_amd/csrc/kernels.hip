@@ -967,9 +967,9 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
                     constexpr int k = K;
                     qw[k] = (mask & (3u << (2 * k + 1))) ? L.rq[(size_t)k * NR + r] : 0u;
                 });
-                T fin[19];
-#pragma unroll
-                for (int i = 0; i < 19; ++i) fin[i] = f[i];
+                // the q < 1/2 rule reads the link's own incoming f_i only when
+                // the opposite link is not walled, and then no link writes f_i:
+                // f[i] still holds it, no copy of the inputs is needed
                 sfor<18>([&](auto I) {
                     constexpr int i = I + 1;
                     constexpr int o = OPP[i];
@@ -985,7 +985,7 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
                         const T cu = cx * uwx + cy * uwy + cz * uwz;
                         T fb;
                         if (q >= (T)0.5) fb = div_(fpi + ((T)2 * q - (T)1) * fpo - (T)6 * wi * cu, (T)2 * q);
-                        else if (!(mask & (1u << o))) fb = (T)2 * q * fpi + ((T)1 - (T)2 * q) * (fin[i] + wi) - (T)6 * wi * cu;
+                        else if (!(mask & (1u << o))) fb = (T)2 * q * fpi + ((T)1 - (T)2 * q) * (f[i] + wi) - (T)6 * wi * cu;
                         else fb = fpi - (T)6 * wi * cu;
                         f[o] = fb - wi;
                         const T flx = (cx - uwx) * fpi + (cx + uwx) * fb;
@@ -1099,6 +1099,26 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
 template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
     bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, blockIdx.y, threadIdx.y * blockDim.x + threadIdx.x);
+}
+
+// One launch per step, one stream: blocks [0, wall_blocks + hw_blocks) do
+// the special cells (dispatched first, they run beside the first bulk
+// blocks), the others SPECIAL_BLOCK / (bx * by) bulk tiles each (tile
+// index = z * xblocks * yblocks + tile in the plane). No second stream and
+// no events per step.
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) k_step(DevLattice L, StepArgs a) {
+    __shared__ double sh[16][6];
+    const uint32_t nsp = L.wall_blocks + L.hw_blocks;
+    if (blockIdx.x < nsp) {
+        special_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, sh);
+        return;
+    }
+    const uint32_t tt = L.bx * L.by;   // divides SPECIAL_BLOCK (checked by the host)
+    const uint32_t g = (blockIdx.x - nsp) * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
+    const uint32_t tiles = L.xblocks * L.yblocks;
+    if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
+    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 // ---------------------------------------------------------------- batches
@@ -1948,6 +1968,29 @@ hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_sp
     return hipGetLastError();
 }
 
+template <class C, int P, bool FORCE, bool SPONGE>
+hipError_t fused_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+    const uint64_t tiles = (uint64_t)L.xblocks * L.yblocks * L.nz;
+    const uint32_t per = SPECIAL_BLOCK / (L.bx * L.by);
+    const uint32_t nb = L.wall_blocks + L.hw_blocks + cdiv(tiles, per);
+    k_step<C, P, FORCE, SPONGE><<<nb, SPECIAL_BLOCK, 0, s>>>(L, a);
+    return hipGetLastError();
+}
+
+template <class C, int P>
+hipError_t fused_launch(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+    const bool force = L.forcing != 0, sponge = L.sponge != 0;
+    if (!force && !sponge) return fused_launch4<C, P, false, false>(L, a, s);
+    if (force && !sponge) return fused_launch4<C, P, true, false>(L, a, s);
+    if (!force && sponge) return fused_launch4<C, P, false, true>(L, a, s);
+    return fused_launch4<C, P, true, true>(L, a, s);
+}
+
+template <class C>
+hipError_t fused_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    return p == 0 ? fused_launch<C, 0>(L, a, s) : fused_launch<C, 1>(L, a, s);
+}
+
 template <class C, int P>
 hipError_t step_launch(const DevLattice &L, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
     const bool force = L.forcing != 0, sponge = L.sponge != 0;
@@ -1998,6 +2041,16 @@ hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, 
         case SIMLAB_STORAGE_FP16: return step_launch_p<CF16>(L, p, a, s_special, s_bulk);
         case SIMLAB_STORAGE_BF16: return step_launch_p<CBF16>(L, p, a, s_special, s_bulk);
         default: return step_launch_p<CF64>(L, p, a, s_special, s_bulk);
+    }
+}
+
+hipError_t launch_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    if (L.bx * L.by > SPECIAL_BLOCK || SPECIAL_BLOCK % (L.bx * L.by) != 0) return hipErrorInvalidConfiguration;
+    switch (L.storage) {
+        case SIMLAB_STORAGE_FP32: return fused_launch_p<CF32>(L, p, a, s);
+        case SIMLAB_STORAGE_FP16: return fused_launch_p<CF16>(L, p, a, s);
+        case SIMLAB_STORAGE_BF16: return fused_launch_p<CBF16>(L, p, a, s);
+        default: return fused_launch_p<CF64>(L, p, a, s);
     }
 }
 

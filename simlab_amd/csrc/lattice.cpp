@@ -113,8 +113,8 @@ struct simlab_lattice {
     uint64_t nhalfway = 0;           // wall cells handled without a record
     uint64_t memory_bytes = 0;
     bool moving = false;
-    hipStream_t stream = nullptr;    // faces, bulk kernel, copies
-    hipStream_t stream2 = nullptr;   // special kernel, concurrent with the bulk
+    hipStream_t stream = nullptr;    // faces, special kernel, copies
+    hipStream_t stream2 = nullptr;   // bulk kernel, concurrent with the special kernel
     hipEvent_t fork = nullptr, join = nullptr;
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
@@ -129,6 +129,7 @@ struct simlab_lattice {
     int adj_axis = 0;
     bool adj_ready = false;         // checkpoints of the last adjoint_forward are valid
     bool concurrent = true;    // k_special on a second stream beside k_bulk
+    bool fused_launch = true;  // one kernel per step when the tile allows
     double *hist_host = nullptr;   // pinned, KMAX x HIST_STRIDE
     bool timing = false;
     std::vector<hipEvent_t> ev;    // 2 per step of a chunk + 2 around the chunk
@@ -169,17 +170,29 @@ const char *validate(const simlab_lattice_setup *s, std::string &msg) {
 
 // Launch shape: whole x-rows per block. bx = nx rounded up to a wave (at most
 // 1024 lanes), by rows so a block has about `target` threads.
-// Default tile: 16-bit storages run best with short tiles (128 x 1: two
-// waves per block, so more blocks share a CU), 32/64-bit ones with whole
-// rows (measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
-// searches per lattice).
-void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override) {
+// Default launch: lattices of up to FUSED_MAX_CELLS cells step with one
+// kernel per step (special and bulk blocks together: no cross-stream wait,
+// which costs about 15 us a step), larger ones with the two kernels on two
+// streams (each at its own register count, so the bulk kernel keeps its
+// occupancy). Tiles: one SPECIAL_BLOCK-thread tile (16-bit: 128 x 2) in the
+// fused kernel; in the split launch 16-bit storages run best with 128 x 1
+// (two waves per block, more blocks per CU), 32/64-bit ones with whole
+// rows. Measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
+// searches tiles and both launches per lattice.
+constexpr uint64_t FUSED_MAX_CELLS = 16u << 20;
+void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override, bool fused) {
     uint32_t bx = ((L.nx + 63) / 64) * 64;
     if (bx > 1024) bx = 1024;
     const bool half = L.storage == SIMLAB_STORAGE_FP16 || L.storage == SIMLAB_STORAGE_BF16;
-    if (half && bx > 128 && !std::getenv("SIMLAB_CFD_BLOCK")) {
-        bx = 128;
-        target_threads = 128;
+    if (!std::getenv("SIMLAB_CFD_BLOCK")) {
+        if (fused) {
+            bx = std::min<uint32_t>(bx, half ? 128 : SPECIAL_BLOCK);
+            target_threads = SPECIAL_BLOCK;
+            if (bx == 192) bx = 128;   // the tile must divide SPECIAL_BLOCK
+        } else if (half && bx > 128) {
+            bx = 128;
+            target_threads = 128;
+        }
     }
     uint32_t by = by_override ? by_override : std::max<uint32_t>(1, target_threads / bx);
     by = std::max<uint32_t>(1, std::min<uint32_t>(by, std::min<uint32_t>(1024 / bx, L.ny)));
@@ -615,7 +628,9 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.npix = lat->npix;
     L.slicedef = lat->slicedef.as<int4>();
     L.sacc = lat->sacc.as<float>();
-    choose_shape(L, default_block_threads(), 0);
+    if (const char *e = std::getenv("SIMLAB_CFD_LAUNCH")) lat->fused_launch = std::string(e) != "split";
+    else lat->fused_launch = n <= FUSED_MAX_CELLS;
+    choose_shape(L, default_block_threads(), 0, lat->fused_launch);
     {
         std::vector<uint32_t> hw;
         for (uint64_t c = 0; c < n; ++c)
@@ -682,13 +697,22 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             } else if (!lat->fused_faces) {
                 hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
             }
-            if (lat->concurrent) {
-                // k_special on the side stream, concurrent with k_bulk
+            const uint32_t tt = lat->L.bx * lat->L.by;
+            if (lat->fused_launch && tt <= SPECIAL_BLOCK && SPECIAL_BLOCK % tt == 0) {
+                // special and bulk blocks in one launch: no cross-stream wait
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
+                hip_check(launch_step_fused(lat->L, par, a, st), "stream-collide kernel");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+            } else if (lat->concurrent) {
+                // k_special on the main stream, k_bulk on the side stream
+                // after the fork: the special kernel is dispatched first, so
+                // its few blocks hold their slots before k_bulk fills the GPU
+                // and it ends within the bulk kernel instead of after it
                 hip_check(hipEventRecord(lat->fork, st), "event");
                 hip_check(hipStreamWaitEvent(lat->stream2, lat->fork, 0), "event wait");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
-                hip_check(launch_stream_collide(lat->L, par, a, lat->stream2, st), "stream-collide kernels");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], lat->stream2), "event");
+                hip_check(launch_stream_collide(lat->L, par, a, st, lat->stream2), "stream-collide kernels");
+                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], lat->stream2), "event");
                 hip_check(hipEventRecord(lat->join, lat->stream2), "event");
                 hip_check(hipStreamWaitEvent(st, lat->join, 0), "event wait");
             } else {
@@ -1007,7 +1031,11 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<uint32_t, uint32_t>> g_tune;   // best (bx, by) per lattice size
+struct Tuned {
+    uint32_t bx, by;
+    bool fused;
+};
+std::map<TuneKey, Tuned> g_tune;   // best tile and launch per lattice size
 
 // tiles to try: x extents of a whole row rounded to a wave and its halves
 // (>= 64), rows per tile so a tile has 64 .. 1024 threads
@@ -1041,7 +1069,8 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
                 std::lock_guard<std::mutex> g(g_tune_mu);
                 auto it = g_tune.find(key);
                 if (it != g_tune.end()) {
-                    set_shape(L, it->second.first, it->second.second);
+                    set_shape(L, it->second.bx, it->second.by);
+                    lat->fused_launch = it->second.fused;
                     if (tile_out) { tile_out[0] = L.bx; tile_out[1] = L.by; }
                     if (step_us_out) *step_us_out = 0.0;
                     return SIMLAB_OK;
@@ -1067,25 +1096,31 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
             const bool timing0 = lat->timing;
             lat->timing = true;
             double best = 1e30;
-            std::pair<uint32_t, uint32_t> best_tile{L.bx, L.by};
+            Tuned best_tile{L.bx, L.by, lat->fused_launch};
             for (const auto &c : tile_candidates(L)) {
-                set_shape(L, c.first, c.second);
-                if (run_steps(lat, nullptr, 2, nullptr, true) != SIMLAB_OK) continue;   // warm-up
-                if (run_steps(lat, nullptr, steps, nullptr, true) != SIMLAB_OK) continue;
-                const double us = lat->last_step_ms * 1e3 / steps;
-                if (us < best) {
-                    best = us;
-                    best_tile = c;
+                for (int fz = 0; fz < 2; ++fz) {
+                    const uint32_t tt = c.first * c.second;
+                    if (fz && (tt > SPECIAL_BLOCK || SPECIAL_BLOCK % tt != 0)) continue;
+                    set_shape(L, c.first, c.second);
+                    lat->fused_launch = fz != 0;
+                    if (run_steps(lat, nullptr, 2, nullptr, true) != SIMLAB_OK) continue;   // warm-up
+                    if (run_steps(lat, nullptr, steps, nullptr, true) != SIMLAB_OK) continue;
+                    const double us = lat->last_step_ms * 1e3 / steps;
+                    if (us < best) {
+                        best = us;
+                        best_tile = Tuned{c.first, c.second, fz != 0};
+                    }
+                    copy(lat->pops.p, save_pops.p, pb);
+                    copy(lat->rfpost.p, save_fpost.p, lat->rfpost.bytes);
+                    copy(lat->rmac.p, save_mac.p, lat->rmac.bytes);
+                    copy(lat->rft.p, save_ft.p, lat->rft.bytes);
+                    lat->steps_done = steps0;
                 }
-                copy(lat->pops.p, save_pops.p, pb);
-                copy(lat->rfpost.p, save_fpost.p, lat->rfpost.bytes);
-                copy(lat->rmac.p, save_mac.p, lat->rmac.bytes);
-                copy(lat->rft.p, save_ft.p, lat->rft.bytes);
-                lat->steps_done = steps0;
             }
             hip_check(hipStreamSynchronize(st), "GPU wait failed");
             lat->timing = timing0;
-            set_shape(L, best_tile.first, best_tile.second);
+            set_shape(L, best_tile.bx, best_tile.by);
+            lat->fused_launch = best_tile.fused;
             {
                 std::lock_guard<std::mutex> g(g_tune_mu);
                 g_tune[key] = best_tile;
@@ -1097,6 +1132,13 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
             return fail(e.code, e.what());
         }
     });
+}
+
+int simlab_cfd_lattice_get_tile(const simlab_lattice *lat, uint32_t *tile_out) {
+    if (!lat || !tile_out) return fail(SIMLAB_ERR_INVALID, "null argument");
+    tile_out[0] = lat->L.bx;
+    tile_out[1] = lat->L.by;
+    return SIMLAB_OK;
 }
 
 int simlab_cfd_lattice_set_tile(simlab_lattice *lat, uint32_t bx, uint32_t by) {
