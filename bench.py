@@ -119,6 +119,86 @@ def large_runs(sa, W, steps, warmup):
     return out
 
 
+def study_prop(sa, W, steps):
+    """The spinning prop at 384^3 (tip speed 0.08, axial inflow 0.01, Cs
+    0.16, smoothstep spin-up over half a revolution): thrust (F_z) and
+    torque (M_z) averaged over the second half of the run, FP16 and FP32
+    storage against FP64 storage (the oracle's arithmetic on the GPU)."""
+    out = {}
+    for name, st in (("fp64", sa.Storage.FP64), ("fp32", sa.Storage.FP32), ("fp16", sa.Storage.FP16)):
+        setup, geom, meta = W.spinning_prop(storage=st)
+        lat = sa.Lattice(setup, geom)
+        drv = W.SpinDriver(meta["axis"], meta["omega"])
+        hs, t0 = [], time.perf_counter()
+        for done in range(0, steps, 2000):
+            hs.append(lat.run(drv.next(min(2000, steps - done))))
+            print(f"# prop {name}: {done + len(hs[-1])} / {steps} steps, {time.perf_counter() - t0:.0f} s",
+                  file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        import numpy as np
+        h = np.concatenate(hs)
+        w = h[steps // 2:]
+        out[name] = {"thrust": float(w[:, 2].mean()), "torque": float(w[:, 5].mean()),
+                     "side_force": float(np.hypot(w[:, 0].mean(), w[:, 1].mean())),
+                     "mlups": round(setup.cells() * steps / wall / 1e6, 1), "revolutions": drv.theta / (2 * 3.141592653589793)}
+        del lat
+    for name in ("fp32", "fp16"):
+        out[name]["thrust_err_vs_fp64"] = out[name]["thrust"] / out["fp64"]["thrust"] - 1.0
+        out[name]["torque_err_vs_fp64"] = out[name]["torque"] / out["fp64"]["torque"] - 1.0
+    out["window"] = f"mean over steps {steps // 2}..{steps}"
+    return out
+
+
+def study_arm(sa, W, steps, batch_steps):
+    """The drone-arm tunnel 512x256x256 (u 0.06, Cs 0.16): drag per storage
+    against FP64; then the arm at 128^3, 192^3, 256^3 and 384^3 in FP16,
+    autotuned, stepped one after the other and as one batch."""
+    import numpy as np
+    out = {}
+    for name, st in (("fp64", sa.Storage.FP64), ("fp32", sa.Storage.FP32), ("fp16", sa.Storage.FP16)):
+        setup, geom, meta = W.drone_arm(storage=st)
+        lat = sa.Lattice(setup, geom)
+        t0 = time.perf_counter()
+        hs = []
+        for done in range(0, steps, 2000):
+            hs.append(lat.run_still(min(2000, steps - done)))
+            print(f"# arm {name}: {done + len(hs[-1])} / {steps} steps", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        h = np.concatenate(hs)
+        out[name] = {"drag": float(h[steps // 2:, 0].mean()), "mlups": round(setup.cells() * steps / wall / 1e6, 1),
+                     "nrec": lat.nrec, "halfway_cells": lat.halfway_cells}
+        del lat
+    for name in ("fp32", "fp16"):
+        out[name]["drag_err_vs_fp64"] = out[name]["drag"] / out["fp64"]["drag"] - 1.0
+    for label, sizes, k in (("batch", (128, 192, 256, 384), batch_steps),
+                            ("batch_small", (64, 80, 96, 112, 128, 64, 96, 128), 4 * batch_steps)):
+        lats, cells, tiles = [], 0, {}
+        for n in sizes:
+            setup, geom, meta = W.drone_arm(dims=(n, n, n), storage=sa.Storage.FP16, scale=n / 256.0)
+            lat = sa.Lattice(setup, geom)
+            t, us = lat.autotune(steps=20, use_cache=True)
+            tiles[str(n)] = {"tile": list(t), "step_us": round(us, 1)}
+            lats.append(lat)
+            cells += setup.cells()
+        for lat in lats:
+            lat.run_still(10)
+        t0 = time.perf_counter()
+        for lat in lats:
+            lat.run_still(k)
+        seq = time.perf_counter() - t0
+        b = sa.Batch(lats)
+        b.run_still(10)
+        t0 = time.perf_counter()
+        b.run_still(k)
+        bat = time.perf_counter() - t0
+        out[label] = {"sizes": list(sizes), "tiles": tiles, "steps": k,
+                      "sequential_mlups": round(cells * k / seq / 1e6, 1),
+                      "batched_mlups": round(cells * k / bat / 1e6, 1)}
+        del b, lats
+    out["window"] = f"mean over steps {steps // 2}..{steps}"
+    return out
+
+
 def drag_run(sa, W, storage, steps):
     setup, geom, meta = W.sphere_channel_large(storage)
     lat = sa.Lattice(setup, geom)
@@ -189,6 +269,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--no-384", action="store_true", help="skip the 384^3 sphere-channel and spinning-prop timings")
+    ap.add_argument("--study", choices=["prop", "arm"], default=None,
+                    help="accuracy studies instead of the benchmark line: the 384^3 spinning prop's thrust and "
+                         "torque per storage (prop), or the 512x256x256 drone arm's drag per storage plus the "
+                         "batched arm at four sizes (arm); one JSON object")
+    ap.add_argument("--study-steps", type=int, default=0, help="steps of a study run (prop 20000, arm 10000)")
     ap.add_argument("--only-384", action="store_true",
                     help="only the 384^3 timings (one JSON object of them; for profiling those kernels alone)")
     ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16)")
@@ -210,6 +295,12 @@ def main():
 
     info = sa.gpu_info()
     print(f"# {info}", file=sys.stderr, flush=True)
+    if args.study == "prop":
+        print(json.dumps({"study_prop384": study_prop(sa, W, args.study_steps or 20000)}), flush=True)
+        return
+    if args.study == "arm":
+        print(json.dumps({"study_arm": study_arm(sa, W, args.study_steps or 10000, 500)}), flush=True)
+        return
     if args.only_384:
         print(json.dumps({"n384": large_runs(sa, W, args.steps, args.warmup)}), flush=True)
         return

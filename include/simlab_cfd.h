@@ -220,15 +220,26 @@ int simlab_cfd_lattice_slice_means(simlab_lattice *lat, float *out);
  * mass check; it is the north star's global conservation check. */
 int simlab_cfd_lattice_mass(simlab_lattice *lat, double *mass);
 
-/* Raw population slots in storage order (19 x cells elements of the storage
- * type: float, uint16 half/bf16 bits, or double), for checkpoints and tests.
+/* Raw population slots in the reference's order (slot of direction i and
+ * cell n at i * cells + n, whatever the device layout; 19 x cells elements
+ * of the storage type: float, uint16 half/bf16 bits, or double), for tests.
  * `bytes` must equal 19 * cells * element size. */
 int simlab_cfd_lattice_get_pops(simlab_lattice *lat, void *out, size_t bytes);
 int simlab_cfd_lattice_set_pops(simlab_lattice *lat, const void *in, size_t bytes);
 
+/* Checkpoint and restore (new: the reference has none). A checkpoint holds
+ * the whole evolving state of a lattice: populations, flags, wall records,
+ * sampler sums and steps_done; restoring it into a lattice built with the
+ * same setup and geometry continues bit for bit as the original would.
+ * checkpoint_size gives the bytes the caller provides. */
+int simlab_cfd_lattice_checkpoint_size(const simlab_lattice *lat, size_t *bytes);
+int simlab_cfd_lattice_checkpoint(simlab_lattice *lat, void *out, size_t bytes);
+int simlab_cfd_lattice_restore(simlab_lattice *lat, const void *in, size_t bytes);
+
 /* Device timing of the last `run` call: total milliseconds of its steps and
- * the summed duration and count of the bulk stream-collide launches,
- * measured with HIP events on the stream they run on (timing must have been
+ * the summed duration and count of the stream-collide launches (the
+ * one-kernel step, or the bulk kernel of the two-stream step), measured
+ * with HIP events on the stream they run on (timing must have been
  * enabled). */
 int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
 int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
@@ -237,11 +248,13 @@ int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
 /* Launch-shape autotuner (new; no counterpart in the reference: wgpu fixes
  * the workgroup size). Times `steps` steps (default 20) of the lattice for
  * each candidate tile of the stream-collide kernel (bx x-cells, a multiple
- * of 64, by rows, 64-1024 threads), keeps the fastest, and restores the
- * lattice's state (populations, records, steps_done) after every trial.
- * With use_cache the tile found earlier in this process for a lattice of the
- * same size, storage and wall presence is taken without timing. tile_out
- * (may be NULL) gets {bx, by}; step_us_out the best step time (0 if cached). */
+ * of 64, by rows, 64-1024 threads) and each launch (one kernel per step,
+ * where the tile divides 256 threads, or two kernels on two streams), keeps
+ * the fastest, and restores the lattice's state (populations, records,
+ * steps_done) after every trial. With use_cache the choice made earlier in
+ * this process for a lattice of the same size, storage and wall presence is
+ * taken without timing. tile_out (may be NULL) gets {bx, by}; step_us_out
+ * the best step time (0 if cached). */
 int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cache, uint32_t *tile_out,
                                 double *step_us_out);
 /* Sets the tile of the stream-collide kernel explicitly. */
@@ -251,10 +264,13 @@ int simlab_cfd_lattice_get_tile(const simlab_lattice *lat, uint32_t *tile_out);
 
 /* Batches (new): several independent lattices (one storage, the same body
  * force and sponge settings, static geometry, the same steps_done) stepped
- * together, one launch per kernel and step for the whole batch; each lattice
- * keeps its own tile. The lattices stay owned by the caller and must outlive
- * the batch. run_still: nsteps still steps of every lattice; ft_out (may be
- * NULL) is lattices x nsteps x 6 doubles, as `run` gives per lattice. */
+ * together, one launch per kernel and step for the whole batch (256-thread
+ * blocks; a lattice's tile is kept if it divides 256). Lattices of more than
+ * 4 Mi cells step with their own launches inside the call (their launch
+ * overhead is small beside their step). The lattices stay owned by the
+ * caller and must outlive the batch. run_still: nsteps still steps of every
+ * lattice; ft_out (may be NULL) is lattices x nsteps x 6 doubles, as `run`
+ * gives per lattice. */
 int simlab_cfd_batch_new(simlab_lattice *const *lats, size_t n, simlab_batch **out);
 int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out);
 void simlab_cfd_batch_free(simlab_batch *b);

@@ -122,6 +122,9 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_lattice_autotune": (I, [V, C.c_uint32, I, P, C.POINTER(D)]),
         "simlab_cfd_lattice_set_tile": (I, [V, C.c_uint32, C.c_uint32]),
         "simlab_cfd_lattice_get_tile": (I, [V, V]),
+        "simlab_cfd_lattice_checkpoint_size": (I, [V, V]),
+        "simlab_cfd_lattice_checkpoint": (I, [V, V, C.c_size_t]),
+        "simlab_cfd_lattice_restore": (I, [V, V, C.c_size_t]),
         "simlab_cfd_batch_new": (I, [P, C.c_size_t, C.POINTER(V)]),
         "simlab_cfd_batch_run_still": (I, [V, C.c_size_t, P]),
         "simlab_cfd_batch_free": (None, [V]),
@@ -582,6 +585,20 @@ class Lattice:
 
     def set_tile(self, bx: int, by: int):
         _check(self._lib, self._lib.simlab_cfd_lattice_set_tile(self._h, int(bx), int(by)))
+
+    def checkpoint(self) -> np.ndarray:
+        """The whole evolving state (populations, flags, records, sampler
+        sums, steps_done) as bytes."""
+        n = C.c_size_t()
+        _check(self._lib, self._lib.simlab_cfd_lattice_checkpoint_size(self._h, C.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        _check(self._lib, self._lib.simlab_cfd_lattice_checkpoint(self._h, _ptr(out), n.value))
+        return out
+
+    def restore(self, ckpt: np.ndarray):
+        """Puts back a checkpoint of a lattice with the same setup and geometry."""
+        c = np.ascontiguousarray(ckpt, np.uint8)
+        _check(self._lib, self._lib.simlab_cfd_lattice_restore(self._h, _ptr(c), c.size))
 
     def tile(self):
         """(bx, by) of the stream-collide kernel's blocks."""

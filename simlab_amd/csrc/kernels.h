@@ -122,8 +122,8 @@ struct StepArgs {
 
 // A batch of lattices stepped together (device arrays prepared by the host):
 // lats[nl] device views, boff_bulk[nl + 1] / boff_sp[nl + 1] first block of
-// each lattice in the bulk / special launch, `threads` per bulk block (each
-// lattice's bx * by fits in it).
+// each lattice in the bulk / special launch, `threads` (SPECIAL_BLOCK) per
+// bulk block (each lattice's bx * by divides it).
 struct BatchLaunch {
     const DevLattice *lats;
     const uint32_t *boff_bulk, *boff_sp;
@@ -131,7 +131,8 @@ struct BatchLaunch {
     int32_t storage;
     bool force, sponge;
 };
-hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a, hipStream_t s);
+// special kernel of the batch on s_special, bulk kernel on s_bulk
+hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
 // standalone face update before a step of this parity (seed of that step)

@@ -1133,14 +1133,20 @@ __device__ __forceinline__ uint32_t batch_member(const uint32_t *boff, uint32_t 
     return __builtin_amdgcn_readfirstlane(i);   // uniform: keeps the lattice's fields in scalar registers
 }
 
+// SPECIAL_BLOCK threads per block; a lattice's tile (bx * by threads, set
+// by the host to divide SPECIAL_BLOCK) fills the block SPECIAL_BLOCK / tile
+// times, so no lattice's blocks idle on another lattice's larger tile
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(1024) k_bulk_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ boff,
-                                                     uint32_t nl, StepArgs a) {
+__global__ void __launch_bounds__(SPECIAL_BLOCK) k_bulk_batch(const DevLattice *__restrict__ lats,
+                                                              const uint32_t *__restrict__ boff, uint32_t nl, StepArgs a) {
     const uint32_t i = batch_member(boff, nl, blockIdx.x);
     const DevLattice &L = lats[i];
     const uint32_t local = __builtin_amdgcn_readfirstlane(blockIdx.x - boff[i]);
+    const uint32_t tt = L.bx * L.by;
+    const uint32_t g = local * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;
-    bulk_block<C, P, FORCE, SPONGE>(L, a, local % tiles, local / tiles, threadIdx.x);
+    if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
+    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 template <class C, int P, bool FORCE, bool SPONGE>
@@ -2165,25 +2171,24 @@ hipError_t launch_adjoint_step(const DevLattice &L, const DevLattice &A, int p, 
 
 namespace {
 template <class C, int P, bool FORCE, bool SPONGE>
-hipError_t batch_launch4(const DevLattice *lats, const uint32_t *boff_bulk, uint32_t nbulk, const uint32_t *boff_sp,
-                         uint32_t nsp, uint32_t nl, uint32_t threads, const StepArgs &a, hipStream_t s) {
-    k_special_batch<C, P, FORCE, SPONGE><<<nsp, SPECIAL_BLOCK, 0, s>>>(lats, boff_sp, nl, a);
+hipError_t batch_launch4(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    k_special_batch<C, P, FORCE, SPONGE><<<b.nsp, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_bulk_batch<C, P, FORCE, SPONGE><<<nbulk, threads, 0, s>>>(lats, boff_bulk, nl, a);
+    k_bulk_batch<C, P, FORCE, SPONGE><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
     return hipGetLastError();
 }
 template <class C, int P>
-hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s) {
-    if (!b.force && !b.sponge) return batch_launch4<C, P, false, false>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
-    if (b.force && !b.sponge) return batch_launch4<C, P, true, false>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
-    if (!b.force && b.sponge) return batch_launch4<C, P, false, true>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
-    return batch_launch4<C, P, true, true>(b.lats, b.boff_bulk, b.nbulk, b.boff_sp, b.nsp, b.nl, b.threads, a, s);
+hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    if (!b.force && !b.sponge) return batch_launch4<C, P, false, false>(b, a, s1, s2);
+    if (b.force && !b.sponge) return batch_launch4<C, P, true, false>(b, a, s1, s2);
+    if (!b.force && b.sponge) return batch_launch4<C, P, false, true>(b, a, s1, s2);
+    return batch_launch4<C, P, true, true>(b, a, s1, s2);
 }
 }  // namespace
 
-hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hipStream_t s) {
-#define BATCH(C) return p == 0 ? batch_launch2<C, 0>(b, a, s) : batch_launch2<C, 1>(b, a, s)
+hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+#define BATCH(C) return p == 0 ? batch_launch2<C, 0>(b, a, s1, s2) : batch_launch2<C, 1>(b, a, s1, s2)
     switch (b.storage) {
         case SIMLAB_STORAGE_FP32: BATCH(CF32);
         case SIMLAB_STORAGE_FP16: BATCH(CF16);

@@ -1134,6 +1134,94 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
     });
 }
 
+}  // extern "C"
+namespace {
+// the buffers a checkpoint holds, in order
+std::vector<DevBuf *> ckpt_bufs(simlab_lattice *lat) {
+    return {&lat->pops, &lat->flags, &lat->rmask, &lat->rq, &lat->rfpost, &lat->rmac, &lat->rft, &lat->vacc, &lat->sacc};
+}
+struct CkptHead {
+    char magic[8];
+    uint32_t dims[3];
+    int32_t storage;
+    uint32_t nrec, nbuf;
+    uint64_t steps_done;
+    uint64_t sizes[9];
+};
+}  // namespace
+extern "C" {
+
+int simlab_cfd_lattice_checkpoint_size(const simlab_lattice *lat, size_t *bytes) {
+    if (!lat || !bytes) return fail(SIMLAB_ERR_INVALID, "null argument");
+    size_t b = sizeof(CkptHead);
+    for (DevBuf *d : ckpt_bufs(const_cast<simlab_lattice *>(lat))) b += d->bytes;
+    *bytes = b;
+    return SIMLAB_OK;
+}
+
+int simlab_cfd_lattice_checkpoint(simlab_lattice *lat, void *out, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!lat || !out) return fail(SIMLAB_ERR_INVALID, "null argument");
+        size_t want = 0;
+        simlab_cfd_lattice_checkpoint_size(lat, &want);
+        if (bytes != want) return fail(SIMLAB_ERR_INVALID, "checkpoint buffer has the wrong size");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            CkptHead h{};
+            std::memcpy(h.magic, "SLCKPT1", 8);
+            for (int k = 0; k < 3; ++k) h.dims[k] = lat->setup.dims[k];
+            h.storage = lat->storage;
+            h.nrec = lat->nrec;
+            const auto bufs = ckpt_bufs(lat);
+            h.nbuf = (uint32_t)bufs.size();
+            h.steps_done = lat->steps_done;
+            char *o = (char *)out + sizeof(CkptHead);
+            for (size_t j = 0; j < bufs.size(); ++j) {
+                h.sizes[j] = bufs[j]->bytes;
+                if (bufs[j]->bytes)
+                    hip_check(hipMemcpyAsync(o, bufs[j]->p, bufs[j]->bytes, hipMemcpyDeviceToHost, lat->stream), "GPU read failed");
+                o += bufs[j]->bytes;
+            }
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            std::memcpy(out, &h, sizeof h);
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+int simlab_cfd_lattice_restore(simlab_lattice *lat, const void *in, size_t bytes) {
+    return guarded([&]() -> int {
+        if (!lat || !in) return fail(SIMLAB_ERR_INVALID, "null argument");
+        size_t want = 0;
+        simlab_cfd_lattice_checkpoint_size(lat, &want);
+        CkptHead h;
+        if (bytes != want || bytes < sizeof h) return fail(SIMLAB_ERR_INVALID, "checkpoint does not fit this lattice");
+        std::memcpy(&h, in, sizeof h);
+        const auto bufs = ckpt_bufs(lat);
+        bool ok = std::memcmp(h.magic, "SLCKPT1", 8) == 0 && h.storage == lat->storage && h.nrec == lat->nrec &&
+                  h.nbuf == bufs.size();
+        for (int k = 0; k < 3; ++k) ok = ok && h.dims[k] == lat->setup.dims[k];
+        for (size_t j = 0; ok && j < bufs.size(); ++j) ok = h.sizes[j] == bufs[j]->bytes;
+        if (!ok) return fail(SIMLAB_ERR_INVALID, "checkpoint does not fit this lattice");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            const char *o = (const char *)in + sizeof(CkptHead);
+            for (DevBuf *d : bufs) {
+                if (d->bytes) hip_check(hipMemcpyAsync(d->p, o, d->bytes, hipMemcpyHostToDevice, lat->stream), "GPU write failed");
+                o += d->bytes;
+            }
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+            lat->steps_done = h.steps_done;
+            lat->adj_ready = false;
+            return SIMLAB_OK;
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
 int simlab_cfd_lattice_get_tile(const simlab_lattice *lat, uint32_t *tile_out) {
     if (!lat || !tile_out) return fail(SIMLAB_ERR_INVALID, "null argument");
     tile_out[0] = lat->L.bx;
@@ -1150,16 +1238,23 @@ int simlab_cfd_lattice_set_tile(simlab_lattice *lat, uint32_t bx, uint32_t by) {
 }
 
 // ---------------------------------------------------------------- batches
+// lattices of more cells than this in a batch step with their own launches
+constexpr uint64_t BATCH_SOLO_CELLS = 4u << 20;
+
 struct simlab_batch {
     std::vector<simlab_lattice *> lats;
     DevBuf dlats, boffs;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;    // faces, special kernel, copies
+    hipStream_t stream2 = nullptr;   // bulk kernel, concurrent with the special kernel
+    hipEvent_t fork = nullptr, join = nullptr;
     int device = 0;
     ~simlab_batch() {
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (stream2) (void)hipStreamSynchronize(stream2);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (stream2) (void)hipStreamDestroy(stream2);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -1187,6 +1282,9 @@ int simlab_cfd_batch_new(simlab_lattice *const *lats, size_t n, simlab_batch **o
             b->dlats.alloc(n * sizeof(DevLattice), mem);
             b->boffs.alloc(2 * (n + 1) * 4, mem);
             hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
             *out = b.release();
             return SIMLAB_OK;
         } catch (const HipError &e) {
@@ -1207,15 +1305,46 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
         try {
             hip_check(hipSetDevice(b->device), "hipSetDevice");
             hipStream_t st = b->stream;
+            // Lattices above BATCH_SOLO_CELLS step with their own launches
+            // (at their own tile and register count; launch overhead is
+            // small beside their step), the others in the batched launch.
+            std::vector<size_t> mi;
+            for (size_t i = 0; i < n; ++i) {
+                simlab_lattice *l = b->lats[i];
+                if (l->ncell <= BATCH_SOLO_CELLS) {
+                    mi.push_back(i);
+                    continue;
+                }
+                const int rc = run_steps(l, nullptr, nsteps, ft_out ? ft_out + i * nsteps * 6 : nullptr, true);
+                if (rc != SIMLAB_OK) return rc;
+            }
+            if (mi.empty()) return SIMLAB_OK;
+            std::vector<simlab_lattice *> ml;
+            for (size_t i : mi) ml.push_back(b->lats[i]);
+            const size_t n = ml.size();   // the batched lattices from here on
             // device views and block offsets as the lattices stand now
             std::vector<DevLattice> hl(n);
             std::vector<uint32_t> boff(2 * (n + 1), 0);
-            uint32_t threads = 64;
+            const uint32_t threads = SPECIAL_BLOCK;
             for (size_t i = 0; i < n; ++i) {
-                hip_check(hipStreamSynchronize(b->lats[i]->stream), "GPU wait failed");
-                hl[i] = b->lats[i]->L;
-                threads = std::max<uint32_t>(threads, hl[i].bx * hl[i].by);
-                boff[i + 1] = boff[i] + hl[i].xblocks * hl[i].yblocks * hl[i].nz;
+                hip_check(hipStreamSynchronize(ml[i]->stream), "GPU wait failed");
+                hl[i] = ml[i]->L;
+                // the batch's blocks have SPECIAL_BLOCK threads: a tile that
+                // does not divide that is replaced (in the batch's copy):
+                // 128 x 1 for 16-bit storage, else whole rows of up to 256
+                // cells (or 128-cell pieces) stacked to 256 threads
+                uint32_t tt = hl[i].bx * hl[i].by;
+                if (tt > threads || threads % tt != 0) {
+                    const bool half = hl[i].storage == SIMLAB_STORAGE_FP16 || hl[i].storage == SIMLAB_STORAGE_BF16;
+                    uint32_t bx = std::min<uint32_t>(((hl[i].nx + 63) / 64) * 64, threads);
+                    if (threads % bx != 0 || (half && bx > 128)) bx = 128;
+                    uint32_t by = 1;
+                    while (!half && by * 2 * bx <= threads && by * 2 <= hl[i].ny) by *= 2;
+                    set_shape(hl[i], bx, by);
+                    tt = bx * by;
+                }
+                const uint64_t tiles = (uint64_t)hl[i].xblocks * hl[i].yblocks * hl[i].nz;
+                boff[i + 1] = boff[i] + (uint32_t)((tiles + threads / tt - 1) / (threads / tt));
                 boff[n + 1 + i + 1] = boff[n + 1 + i] + hl[i].wall_blocks + hl[i].hw_blocks;
             }
             hip_check(hipMemcpyAsync(b->dlats.p, hl.data(), n * sizeof(DevLattice), hipMemcpyHostToDevice, st), "upload");
@@ -1233,7 +1362,7 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
             bl.sponge = hl[0].sponge != 0;
             simlab_step_in idn{};
             for (int k = 0; k < 3; ++k) idn.rot[k][k] = idn.rot_prev[k][k] = 1.0;
-            const uint64_t step0 = b->lats[0]->steps_done;
+            const uint64_t step0 = ml[0]->steps_done;
             size_t done = 0;
             while (done < nsteps) {
                 const size_t chunk = std::min<size_t>(KMAX, nsteps - done);
@@ -1249,10 +1378,16 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                             hip_check(launch_faces(hl[i], par, a.seed, st), "face kernel");
                             hip_check(launch_faces(hl[i], 1 - par, (uint32_t)(step + 1) * 2654435761u, st), "face kernel");
                         }
-                    hip_check(launch_batch_step(bl, par, a, st), "batched stream-collide kernels");
+                    // special kernel first on the main stream, bulk kernel on
+                    // the side stream beside it (as a lattice's own split step)
+                    hip_check(hipEventRecord(b->fork, st), "event");
+                    hip_check(hipStreamWaitEvent(b->stream2, b->fork, 0), "event wait");
+                    hip_check(launch_batch_step(bl, par, a, st, b->stream2), "batched stream-collide kernels");
+                    hip_check(hipEventRecord(b->join, b->stream2), "event");
+                    hip_check(hipStreamWaitEvent(st, b->join, 0), "event wait");
                 }
                 for (size_t i = 0; i < n; ++i) {
-                    simlab_lattice *l = b->lats[i];
+                    simlab_lattice *l = ml[i];
                     hip_check(launch_reduce_hist(hl[i], (uint32_t)chunk, st), "force reduction kernel");
                     hip_check(hipMemcpyAsync(l->hist_host, l->hist.p, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st),
                               "force copy");
@@ -1262,10 +1397,10 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                     for (size_t i = 0; i < n; ++i)
                         for (size_t k = 0; k < chunk; ++k)
                             for (int j = 0; j < 6; ++j)
-                                ft_out[(i * nsteps + done + k) * 6 + j] = b->lats[i]->hist_host[k * HIST_STRIDE + j];
+                                ft_out[(mi[i] * nsteps + done + k) * 6 + j] = ml[i]->hist_host[k * HIST_STRIDE + j];
                 done += chunk;
             }
-            for (size_t i = 0; i < n; ++i) b->lats[i]->steps_done += nsteps;
+            for (size_t i = 0; i < n; ++i) ml[i]->steps_done += nsteps;
             return SIMLAB_OK;
         } catch (const HipError &e) {
             return fail(e.code, e.what());

@@ -77,3 +77,21 @@ def test_autotune_keeps_state_and_results():
     other = sa.Lattice(s, g)
     t2, us2 = other.autotune(use_cache=True)
     assert t2 == tile and us2 == 0.0
+
+
+def test_batch_with_a_large_member():
+    # a member of more than 4 Mi cells steps with its own launches inside
+    # the batch call: bit-identical to running it alone; the small one goes
+    # through the batched launch
+    big = W.sphere_channel((192, 160, 144), 12.0, 48.0, 200.0, 0.05, 0.1, sa.Storage.FP16, subdiv=3)
+    small = cases(sa.Storage.FP16)[0]
+    say("case batch large member", big[0].dims, small[0].dims)
+    solo = sa.Lattice(big[0], big[1])
+    h_solo = solo.run_still(20)
+    ms = [sa.Lattice(big[0], big[1]), sa.Lattice(small[0], small[1])]
+    ft = sa.Batch(ms).run_still(20)
+    assert ms[0].steps_done == 20 and ms[1].steps_done == 20
+    assert np.array_equal(ft[0], h_solo) and np.array_equal(ms[0].get_pops(), solo.get_pops())
+    ref = sa.Lattice(small[0], small[1])
+    h_ref = ref.run_still(20)
+    assert np.allclose(ft[1], h_ref, rtol=1e-4, atol=1e-4 * np.abs(h_ref).max())
