@@ -815,20 +815,30 @@ template <class T> struct Lat {
     // lattice.rs:675-686 per-step seed
     uint32_t seed_of(uint64_t step) const { return (uint32_t)step * 2654435761u; }
 
-    // lattice.rs:715-738: one step
-    void step(const simlab_step_in &si, double ft[6]) {
-        int p = (int)(steps_done % 2);
+    // lattice.rs:675-686: the step's uniform (rotation columns as f32, seed)
+    StepU stepu(const simlab_step_in &si) const {
         StepU s;
         for (int k = 0; k < 3; ++k)
             for (int r = 0; r < 3; ++r) { s.r[k][r] = (float)si.rot[r][k]; s.q[k][r] = (float)si.rot_prev[r][k]; }
         for (int k = 0; k < 3; ++k) s.omega[k] = (float)si.omega[k];
         s.seed = seed_of(steps_done);
+        return s;
+    }
+    // the re-voxelization stage of a step (lattice.rs:720-726)
+    void spin_stage(const simlab_step_in &si) {
+        int p = (int)(steps_done % 2);
+        StepU s = stepu(si);
+        tri_scatter(s);
+        for (uint32_t k = 0; k < (uint32_t)recs.size(); ++k) voxelize_cell(k, p, s);
+        qclear();
+    }
+
+    // lattice.rs:715-738: one step
+    void step(const simlab_step_in &si, double ft[6]) {
+        int p = (int)(steps_done % 2);
+        StepU s = stepu(si);
         bool moving = geo.moving != 0;
-        if (moving) {
-            tri_scatter(s);
-            for (uint32_t k = 0; k < (uint32_t)recs.size(); ++k) voxelize_cell(k, p, s);
-            qclear();
-        }
+        if (moving) spin_stage(si);
         int64_t nf = (int64_t)faces.size();
         if (!fused || first_of_run) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -1067,6 +1077,15 @@ int oracle_run_still(void *hp, uint64_t nsteps, double *ft) {
     DISPATCH(h, L->first_of_run = true);
     DISPATCH(h, for (uint64_t k = 0; k < nsteps; ++k) L->step(s, ft ? ft + 6 * k : tmp));
     return 0;
+}
+
+// test hook: only the re-voxelization stage of the next step (a spinning
+// part's tri_scatter, voxelize and qclear), without the stream-collide
+int oracle_spin_stage(void *hp, const simlab_step_in *si) {
+    Handle *h = (Handle *)hp;
+    int ok = 1;
+    DISPATCH(h, if (L->geo.moving) L->spin_stage(*si); else ok = 0);
+    return ok ? 0 : -1;
 }
 
 uint64_t oracle_steps_done(void *hp) { Handle *h = (Handle *)hp; uint64_t r = 0; DISPATCH(h, r = L->steps_done); return r; }

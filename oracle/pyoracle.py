@@ -40,6 +40,7 @@ def lib():
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_run.argtypes = [V, C.POINTER(sa._StepIn), C.c_uint64, P]
         L.oracle_run_still.argtypes = [V, C.c_uint64, P]
+        L.oracle_spin_stage.argtypes = [V, C.POINTER(sa._StepIn)]
         L.oracle_steps_done.restype = C.c_uint64
         L.oracle_steps_done.argtypes = [V]
         for n in ("oracle_nrec", "oracle_nface"):
@@ -118,6 +119,12 @@ class OracleLattice:
         arr = sa.steps_to_c(steps)
         self._lib.oracle_run(self._h, arr, len(steps), _ptr(out))
         return out
+
+    def spin_stage(self, step):
+        """Only the re-voxelization stage of the next step (spinning part)."""
+        arr = sa.steps_to_c([step])
+        if self._lib.oracle_spin_stage(self._h, arr) != 0:
+            raise sa.SimlabError("no spinning part")
 
     def run_still(self, n):
         out = np.zeros((int(n), 6), np.float64)

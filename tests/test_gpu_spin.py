@@ -145,3 +145,35 @@ def test_rotating_cube_torque_moving_matches_static():
     say(f"case rotating plate: Mz moving {mz_m:.5e}, static {mz_s:.5e}, ratio {mz_m / mz_s:.4f}")
     assert mz_m < 0.0 and mz_s < 0.0
     assert abs(mz_m / mz_s - 1.0) < 0.05
+
+
+def test_spinning_prop_half_storage_matches_oracle():
+    # FP16 storage with the same dither stream as the oracle: the same walls
+    # and q bits every step; after one step the stored values are identical
+    # but for FP32 rounding differences (one unit in the last place)
+    setup, geom, meta = small_prop(sa.Storage.FP16)
+    say("case spin FP16", setup.dims)
+    lat = sa.Lattice(setup, geom)
+    ora = O.OracleLattice(setup, geom, store=O.F16)
+    drv = W.SpinDriver((0.0, 0.0, 1.0), meta["omega"], ramp_revs=0.0)
+    for n in (1, 9):
+        st = drv.next(n)
+        hg, ho = lat.run(st), ora.run(st)
+        rg, ro = lat.records(), ora.records()
+        assert np.array_equal(rg[:, :11], ro[:, :11])
+        # post-collision outputs of the colliding cells (face-cell slots are
+        # written one step early by the library)
+        fluid = np.nonzero((ora.flags() & 3) == 0)[0]
+        _, smap = O.aa_maps(setup.dims, (ora.steps_done - 1) % 2)
+        idx = smap[fluid].astype(np.int64)
+        pg, po = lat.get_pops()[idx], ora.get_pops()[idx]
+        same = np.mean(pg == po)
+        dv = np.max(np.abs(pg.view(np.float16).astype(np.float64) - po.view(np.float16).astype(np.float64))) / 32768.0
+        df = np.max(np.abs(hg - ho)) / np.max(np.abs(ho))
+        say(f"  after {n} more steps: identical stored values {same:.5f}, max decoded diff {dv:.2e}, "
+            f"force/torque max rel diff {df:.2e}")
+        # one step: bits; ten: one-ulp differences spread (the flow is not
+        # bit-reproducible across arithmetic orders) but stay at ulp size
+        if n == 1:
+            assert same > 0.999
+        assert dv < 2e-5 and df < 2e-5   # measured: 3.8e-6 and 3.1e-6 after 10 steps

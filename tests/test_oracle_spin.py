@@ -58,3 +58,58 @@ def test_rotation_matrix_and_spin_driver():
     assert 0.0 < f[0] < 1e-4 and all(b >= a for a, b in zip(f, f[1:])) and f[-1] == 1.0
     assert np.allclose(steps[1].rot_prev, steps[0].rot)
     assert np.isclose(d.theta, sum(s.omega[2] for s in steps))
+
+
+CV = np.array([(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+               (1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, -1, -1), (1, -1, 0), (-1, 1, 0), (1, 0, -1), (-1, 0, 1),
+               (0, 1, -1), (0, -1, 1)], np.float64)
+WQ = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def test_refill_and_momentum_of_uncovered_and_covered_cells():
+    # shader.rs:1040-1100 on a plate turned by 0.35 rad in one step through
+    # fluid in uniform motion u0 (records at equilibrium, rho = 1): a cell
+    # the plate uncovers gets feq(rho = 1, u_w = omega x r) written as its
+    # previous outputs and books F = -rho u_w; a cell it covers books the
+    # momentum of its fluid, F = rho u0; both with T = r x F
+    from simlab_amd.shapes import cuboid
+    n, u0, om, th = 32, 0.02, 0.01, 0.35
+    c = (15.5, 16.0, 15.5)
+    m = cuboid([-6.0, -1.5, -1.5], [6.0, 1.5, 1.5])
+    vox = sa.voxelize_spin(m.positions, m.triangles, (0.0, 0.0, 1.0), 0.125)
+    geom = sa.Geometry.spin(vox, (5, 5, 11), (22, 22, 10))
+    setup = sa.LatticeSetup(dims=(n, n, n), faces=(sa.FaceBc.PERIODIC,) * 6, nu=0.05, u_inf=(u0, 0.0, 0.0),
+                            u_init=(u0, 0.0, 0.0), storage=sa.Storage.FP64, center=c)
+    ora = O.OracleLattice(setup, geom, store=O.F64)
+    ora.spin_stage(sa.StepIn(sa.IDENTITY, sa.IDENTITY, (0.0, 0.0, 0.0), False))
+    s0 = set(np.nonzero((ora.flags() & 3) == 1)[0].tolist())
+    rot = W.rotation((0.0, 0.0, 1.0), th)
+    ora.spin_stage(sa.StepIn(rot, sa.IDENTITY, (0.0, 0.0, om), False))
+    s1 = set(np.nonzero((ora.flags() & 3) == 1)[0].tolist())
+    unc, cov = sorted(s0 - s1), sorted(s1 - s0)
+    print(f"solid {len(s0)} -> {len(s1)}: {len(unc)} uncovered, {len(cov)} covered")
+    assert len(unc) > 10 and len(cov) > 10
+    recs = ora.records()
+    by_cell = {int(r[1]): r[33:39].view(np.float32).astype(np.float64) for r in recs}
+    pops = ora.get_pops().astype(np.float64)
+    _, st = O.aa_maps(setup.dims, 1)   # the refill goes where a parity-1 step would have put its outputs
+
+    def pc_of(cell):
+        x, y, z = cell % n, (cell // n) % n, cell // (n * n)
+        return np.array([np.float32(x - np.float32(c[0])), np.float32(y - np.float32(c[1])), np.float32(z - np.float32(c[2]))],
+                        np.float64)
+
+    for cell in cov:
+        ft, pc = by_cell[cell], pc_of(cell)
+        F = np.array([u0, 0.0, 0.0])
+        assert np.allclose(ft[:3], F, atol=1e-7) and np.allclose(ft[3:], np.cross(pc, F), atol=1e-6)
+    for cell in unc:
+        pc = pc_of(cell)
+        uw = np.array([np.float32(-np.float32(om) * np.float32(pc[1])), np.float32(np.float32(om) * np.float32(pc[0])), 0.0],
+                      np.float64)
+        ft = by_cell[cell]
+        assert np.allclose(ft[:3], -uw, atol=1e-7) and np.allclose(ft[3:], np.cross(pc, -uw), atol=1e-6)
+        cu = CV @ uw
+        feq = WQ * (3.0 * cu + 4.5 * cu * cu - 1.5 * uw @ uw)
+        got = pops[st[cell].astype(np.int64)]
+        assert np.max(np.abs(got - feq)) < 1e-12, (cell, got - feq)
