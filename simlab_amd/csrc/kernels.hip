@@ -949,24 +949,9 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
             const T pcx = (T)x - (T)L.center[0], pcy = (T)y - (T)L.center[1], pcz = (T)z - (T)L.center[2];
             const T omx = (T)a.omega[0], omy = (T)a.omega[1], omz = (T)a.omega[2];
             if (mask) {
-                // issue every record load this cell needs before using any, so
-                // they are in flight together (the link loop would otherwise
-                // wait for each in turn)
-                uint32_t need = 0;
-                sfor<18>([&](auto I) {
-                    constexpr int i = I + 1;
-                    if (mask & (1u << i)) need |= (1u << i) | (1u << OPP[i]);
-                });
-                T fp[19];
-                uint32_t qw[9];
-                sfor<18>([&](auto I) {
-                    constexpr int i = I + 1;
-                    fp[i] = (need & (1u << i)) ? rfpost[(size_t)(i - 1) * NR + r] : (T)0;
-                });
-                sfor<9>([&](auto K) {
-                    constexpr int k = K;
-                    qw[k] = (mask & (3u << (2 * k + 1))) ? L.rq[(size_t)k * NR + r] : 0u;
-                });
+                // record loads go per link: few registers, so the one-kernel
+                // step keeps the bulk tiles' occupancy; the latency is hidden
+                // behind the bulk blocks that run beside these
                 // the q < 1/2 rule reads the link's own incoming f_i only when
                 // the opposite link is not walled, and then no link writes f_i:
                 // f[i] still holds it, no copy of the inputs is needed
@@ -975,10 +960,11 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
                     constexpr int o = OPP[i];
                     if (mask & (1u << i)) {
                         const T wi = wgt<T>(i);
-                        const uint16_t qh = (uint16_t)(((i - 1) & 1) ? (qw[(i - 1) >> 1] >> 16) : (qw[(i - 1) >> 1] & 0xffffu));
+                        const uint32_t qwi = L.rq[(size_t)((i - 1) >> 1) * NR + r];
+                        const uint16_t qh = (uint16_t)(((i - 1) & 1) ? (qwi >> 16) : (qwi & 0xffffu));
                         const T q = fmax((T)(float)__builtin_bit_cast(_Float16, qh), (T)0.001);
-                        const T fpi = fp[i] + wi;
-                        const T fpo = fp[o] + wi;
+                        const T fpi = rfpost[(size_t)(i - 1) * NR + r] + wi;
+                        const T fpo = rfpost[(size_t)(o - 1) * NR + r] + wi;
                         const T cx = (T)CX[i], cy = (T)CY[i], cz = (T)CZ[i];
                         const T xwx = pcx + q * cx, xwy = pcy + q * cy, xwz = pcz + q * cz;
                         const T uwx = omy * xwz - omz * xwy, uwy = omz * xwx - omx * xwz, uwz = omx * xwy - omy * xwx;
@@ -1106,8 +1092,16 @@ __global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
 // blocks), the others SPECIAL_BLOCK / (bx * by) bulk tiles each (tile
 // index = z * xblocks * yblocks + tile in the plane). No second stream and
 // no events per step.
+// parity 1 with FP16 or FP32 storage and neither body force nor sponge: at
+// least 5 waves per SIMD (96 VGPRs), as the bulk kernel's parity 1 has (the
+// compiler fits them without spilling); the other variants would spill and
+// keep their natural count
+template <class C, int P, bool FORCE, bool SPONGE> constexpr int step_waves() {
+    return ((std::is_same_v<C, CF16> || std::is_same_v<C, CF32>) && P == 1 && !FORCE && !SPONGE) ? 5 : 1;
+}
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(SPECIAL_BLOCK) k_step(DevLattice L, StepArgs a) {
+__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE>())))
+k_step(DevLattice L, StepArgs a) {
     __shared__ double sh[16][6];
     const uint32_t nsp = L.wall_blocks + L.hw_blocks;
     if (blockIdx.x < nsp) {
