@@ -14,7 +14,9 @@
 // (crates/cfd/tests/channel.rs, momentum.rs), closed forms (tau = 1 gives
 // feq, halfway bounce-back, Poiseuille), invariants (mass, zero force on a
 // uniform state), the committed golden vectors, and brute force on tiny
-// lattices. Functions with no such check say "parity unpinned" below.
+// lattices. Functions with no such check say "parity unpinned" below: the
+// sponge and absorbing layer, the Spalding wall model, the samplers, and the
+// mean non-equilibrium stress of the spinning part's refill.
 
 #include "../include/simlab_cfd.h"
 
@@ -348,7 +350,8 @@ template <class T> struct Lat {
         }
     }
 
-    // ---- shader.rs:528-538 ----
+    // ---- shader.rs:528-538 ---- (parity unpinned: sponge ramp and absorbing
+    // layer are checked only against the GPU, no closed form or reference test)
     T sponge(uint32_t x, uint32_t y, uint32_t z) const {
         T p[3] = {(T)x, (T)y, (T)z};
         T hi[3] = {(T)(NX - 1) - p[0], (T)(NY - 1) - p[1], (T)(NZ - 1) - p[2]};
@@ -365,7 +368,8 @@ template <class T> struct Lat {
         return half_to_float((uint16_t)(((i - 1) % 2) ? (w >> 16) : (w & 0xffff)));
     }
 
-    // ---- shader.rs:546-597 wall_tau (Spalding) ----
+    // ---- shader.rs:546-597 wall_tau (Spalding) ---- (parity unpinned: checked
+    // only against the GPU; the reference's tests do not run the wall model)
     T wall_tau(const Rec<T> &r, uint32_t mask, V3<T> pc, V3<T> u, T tau_les, const V3<T> &om) const {
         V3<T> nsum = {0, 0, 0};
         for (int i = 1; i < 19; ++i)
@@ -742,6 +746,9 @@ template <class T> struct Lat {
                     }
                 }
             }
+            // (the mean non-equilibrium stress below is parity unpinned: zero in
+            // the refill test's equilibrium flow, checked otherwise only
+            // against the GPU)
             T rho = cnt > 0 ? rs / cnt : (T)1;
             V3<float> uwf = cross(V3<float>{s.omega[0], s.omega[1], s.omega[2]}, pc);
             V3<T> uw = {(T)uwf.x, (T)uwf.y, (T)uwf.z};
@@ -765,6 +772,7 @@ template <class T> struct Lat {
     }
 
     // ---- shader.rs:736-784 samplers (module parity p) ----
+    // (parity unpinned: the samplers are checked only against the GPU)
     void sample_verts(int p, const StepU &s) {
         size_t nv = verts.size() / 8;
         V3<float> c = centerf();
