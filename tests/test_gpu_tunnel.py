@@ -1,0 +1,87 @@
+"""Whole studies through the study driver (simlab_amd.tunnel.run_with), as
+crates/cfd/tests/tunnel.rs runs them through `run_with`: the sphere at
+Re 100 on 16 cells across against the drag correlations, the test propeller
+in hover (thrust along +axis, torque against the rotation, shaft power
+positive, for either spin direction) and a slowly spinning sphere against
+Stokes' torque. Every run here is on the uniform D3Q19 grid, which is the
+grid the reference uses for these cases too (under 48 cells across, and for
+every spinning part)."""
+import math
+
+import numpy as np
+import pytest
+
+from simlab_amd import tunnel as T
+from simlab_amd.shapes import PropSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_sphere_re100_quick():
+    # tunnel.rs tests: sphere_re100_quick = sphere_check(100, 16, 15, 0.05)
+    _say("study: sphere Re 100, 16 cells across, 15 passes")
+    cd, r = T.sphere_drag(100.0, 16, 15.0)
+    s = r.summary
+    refc = 0.5 * (T.sphere_cd_cgw(100.0) + T.sphere_cd_sn(100.0))
+    _say(f"  Cd {cd:.4f} vs {refc:.4f} ({(cd / refc - 1) * 100:+.2f} %), converged {s.converged}, "
+         f"{s.simulated:.1f} passes, {s.mlups:.0f} MLUPS, {s.seconds:.1f} s")
+    assert abs(cd / refc - 1.0) < 0.05
+    side = math.hypot(s.lift_n, s.side_n) / abs(s.drag_n)
+    assert side < 0.05
+    # the summary's own Cd uses the rasterized frontal area (within 0.1 % of pi D^2 / 4)
+    assert s.cd == pytest.approx(cd, rel=2e-3)
+    assert s.headline.startswith("Drag ") and "Cd" in s.headline
+    assert s.grid == (137, 113, 113) and s.cells == 137 * 113 * 113
+    # surface pressure: stagnation near +q at the front, finite on most vertices
+    cp = r.surface_cp[np.isfinite(r.surface_cp)]
+    assert cp.size > 0.9 * r.surface_cp.size
+    assert 0.7 < cp.max() < 1.3
+    along = next(f for f in r.slices if f.name == "along")
+    assert (along.nu, along.nv) == (135, 111)
+    sp = along.field("speed").data
+    assert np.isfinite(sp).sum() > 0.9 * sp.size
+    # far-field air moves at about the wind speed
+    assert abs(np.nanmedian(sp) / 10.0 - 1.0) < 0.1
+
+
+def _hover(rpm, cells, revs):
+    _say(f"study: 5-inch prop hover, {rpm:+.0f} rpm, {cells} cells across, {revs} revolutions")
+    r = T.prop_hover(PropSpec.five_inch(), rpm, cells, revs)
+    s = r.summary
+    _say(f"  thrust {s.thrust_n:.4f} N, torque {s.torque_nm:.5f} N m, power {s.power_w:.2f} W, {s.steps} steps, "
+         f"{s.seconds:.1f} s")
+    return r
+
+
+def test_reversed_spin_mirrors_torque():
+    # tunnel.rs tests: reversed_spin_mirrors_torque
+    a = _hover(20000.0, 24, 1.5).summary
+    b = _hover(-20000.0, 24, 1.5).summary
+    assert a.torque_nm < 0.0 and b.torque_nm > 0.0
+    assert a.power_w > 0.0 and b.power_w > 0.0
+
+
+def test_spinning_prop_smoke():
+    # tunnel.rs tests: spinning_prop_smoke
+    r = _hover(20000.0, 32, 2.0)
+    s = r.summary
+    assert s.thrust_n > 0.0
+    assert s.torque_nm < 0.0
+    assert s.power_w > 0.0
+    assert np.isfinite(r.surface_pressure_pa).sum() > r.surface_pressure_pa.size // 2
+    assert s.headline.startswith("Thrust ") and "rpm" in s.headline
+
+
+def test_rotating_sphere_stokes_torque():
+    # tunnel.rs tests: rotating_sphere_stokes_torque (side margin 2 rotor diameters)
+    _say("study: sphere spinning at Re 1, 24 cells across, 6 revolutions")
+    o = T.SolverOptions()
+    o.still_margins = T.Margins(2.5, 2.5, 2.0)
+    t, stokes, r = T.rotating_sphere(1.0, 24, 6.0, o)
+    _say(f"  torque {-t:.4e} vs 8 pi mu R^3 omega {stokes:.4e} ({(-t / stokes - 1) * 100:+.2f} %), "
+         f"thrust {r.summary.thrust_n:.2e}, {r.summary.steps} steps, {r.summary.seconds:.1f} s")
+    assert abs(-t / stokes - 1.0) < 0.05
