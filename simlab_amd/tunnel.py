@@ -803,6 +803,88 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         lat.close()
 
 
+# ---------------------------------------------------------------- result files (engine wind.rs:12-15, 440-497)
+def _range(v: np.ndarray):
+    """wind.rs:452-461: finite min and max, (0, 0) when none."""
+    f = v[np.isfinite(v)]
+    return (float(f.min()), float(f.max())) if f.size else (0.0, 0.0)
+
+
+def safe_name(name: str) -> bool:
+    """engine api.rs:392-394."""
+    return bool(name) and all(c.isascii() and (c.isalnum() or c in "_.-") for c in name) and ".." not in name
+
+
+def write_display_mesh(directory: str, prefix: str, m: TriMesh, patch=None) -> None:
+    """core mesh.rs:176-198: positions f32 x3, area-weighted vertex normals
+    f32 x3, indices u32 x3, patch u32 per triangle (0 when none), little-endian."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    P = np.asarray(m.positions, np.float64)
+    vn = _vertex_normals(m) if len(m.triangles) else np.zeros_like(P)
+    np.asarray(P, "<f4").tofile(os.path.join(directory, f"{prefix}.positions.f32"))
+    np.asarray(vn, "<f4").tofile(os.path.join(directory, f"{prefix}.normals.f32"))
+    np.asarray(m.triangles, "<u4").reshape(-1).tofile(os.path.join(directory, f"{prefix}.indices.u32"))
+    pt = np.zeros(len(m.triangles), "<u4") if patch is None else np.asarray(patch, "<u4")
+    pt.tofile(os.path.join(directory, f"{prefix}.patch.u32"))
+
+
+def write_result(directory: str, r: WindResult, surface_mm: Optional[TriMesh] = None):
+    """The result files of a run (wind.rs:12-15, 465-497): the model surface
+    as mesh.* (when given, in model mm), field.pressure.f32 and field.cp.f32
+    per surface vertex, and each slice as slice.<name>.json plus
+    slice.<name>.<field>.f32 (v-major rows of nu values, NaN inside the part).
+    Returns the run's (fields, slices) lists."""
+    import json
+    import os
+    os.makedirs(directory, exist_ok=True)
+    if surface_mm is not None:
+        write_display_mesh(directory, "mesh", surface_mm)
+    fields = []
+    pres = np.asarray(r.surface_pressure_pa, "<f4")
+    if np.isfinite(pres).any():
+        lo, hi = _range(pres)
+        pres.tofile(os.path.join(directory, "field.pressure.f32"))
+        m = max(abs(lo), abs(hi))
+        fields.append({"name": "pressure", "label": "Air pressure", "unit": "Pa", "min": -m, "max": m,
+                       "data_min": lo, "data_max": hi, "scale": "linear", "diverging": True})
+    cp = np.asarray(r.surface_cp, "<f4")
+    if np.isfinite(cp).any():
+        lo, hi = _range(cp)
+        cp.tofile(os.path.join(directory, "field.cp.f32"))
+        fields.append({"name": "cp", "label": "Pressure coefficient", "unit": "", "min": lo, "max": hi,
+                       "scale": "linear", "diverging": True})
+    slices = []
+    for sl in r.slices:
+        if not sl.fields:
+            continue
+        infos = []
+        for f in sl.fields:
+            if not safe_name(f.name):
+                continue
+            lo, hi = _range(np.asarray(f.data))
+            np.asarray(f.data, "<f4").tofile(os.path.join(directory, f"slice.{sl.name}.{f.name}.f32"))
+            infos.append({"name": f.name, "label": f.label, "unit": f.unit, "min": lo, "max": hi})
+        meta = {"name": sl.name, "label": sl.label, "origin_mm": [float(x) for x in sl.origin_mm],
+                "u_mm": [float(x) for x in sl.u_mm], "v_mm": [float(x) for x in sl.v_mm], "nu": int(sl.nu),
+                "nv": int(sl.nv), "fields": infos}
+        with open(os.path.join(directory, f"slice.{sl.name}.json"), "w") as fh:
+            json.dump(meta, fh, separators=(",", ":"))
+        slices.append({"name": sl.name, "label": sl.label})
+    return fields, slices
+
+
+def read_slice(directory: str, name: str):
+    """Reads slice.<name>.json and its fields back: (meta, {field: (nv, nu) array})."""
+    import json
+    import os
+    with open(os.path.join(directory, f"slice.{name}.json")) as fh:
+        meta = json.load(fh)
+    data = {f["name"]: np.fromfile(os.path.join(directory, f"slice.{name}.{f['name']}.f32"), "<f4").reshape(
+        meta["nv"], meta["nu"]) for f in meta["fields"]}
+    return meta, data
+
+
 # ---------------------------------------------------------------- reference cases (validation.rs)
 SPHERE_D = 0.05
 SPHERE_U = 10.0

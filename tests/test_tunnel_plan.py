@@ -168,3 +168,47 @@ def test_slender_part_sizing_on_a_uniform_grid():
         assert e.cells_across >= T.cells_across(q)
         if e.cells_thin < target - 0.5:
             assert e.thin_warning is not None and f"{want} cells" in e.thin_warning
+
+
+def test_result_files_round_trip(tmp_path):
+    # engine wind.rs:465-497 file set, written from a synthetic result
+    nu, nv = 5, 3
+    data = np.arange(nu * nv, dtype=np.float32)
+    data[4] = np.nan
+    sl = T.FlowSlice("along", "Along the wind", np.array([1.0, 2.0, 3.0]), np.array([50.0, 0, 0]),
+                     np.array([0, 0, 30.0]), nu, nv,
+                     [T.SliceField("speed", "Air speed", "m/s", 0.0, 14.0, data),
+                      T.SliceField("../bad", "x", "", 0.0, 0.0, data)])
+    pres = np.array([-3.0, 1.0, np.nan, 2.0], np.float32)
+    r = T.WindResult(None, pres, pres / 2.0, [sl], [])
+    m = cuboid([0, 0, 0], [1, 2, 3])
+    fields, slices = T.write_result(str(tmp_path), r, m)
+    assert [f["name"] for f in fields] == ["pressure", "cp"]
+    # pressure range symmetric about zero (white is ambient), data range kept
+    assert fields[0]["min"] == -3.0 and fields[0]["max"] == 3.0
+    assert fields[0]["data_min"] == -3.0 and fields[0]["data_max"] == 2.0
+    got = np.fromfile(tmp_path / "field.pressure.f32", "<f4")
+    assert np.array_equal(got, pres, equal_nan=True)
+    assert slices == [{"name": "along", "label": "Along the wind"}]
+    meta, d = T.read_slice(str(tmp_path), "along")
+    assert meta["nu"] == nu and meta["nv"] == nv and meta["origin_mm"] == [1.0, 2.0, 3.0]
+    # unsafe field names are skipped
+    assert [f["name"] for f in meta["fields"]] == ["speed"]
+    assert meta["fields"][0]["min"] == 0.0 and meta["fields"][0]["max"] == 14.0
+    assert np.array_equal(d["speed"].reshape(-1), data, equal_nan=True)
+    assert not (tmp_path / "slice.along.../bad.f32").exists()
+    # display mesh: positions, unit normals, indices, one patch per triangle
+    pos = np.fromfile(tmp_path / "mesh.positions.f32", "<f4").reshape(-1, 3)
+    nrm = np.fromfile(tmp_path / "mesh.normals.f32", "<f4").reshape(-1, 3)
+    idx = np.fromfile(tmp_path / "mesh.indices.u32", "<u4").reshape(-1, 3)
+    patch = np.fromfile(tmp_path / "mesh.patch.u32", "<u4")
+    assert np.array_equal(pos, np.asarray(m.positions, np.float32))
+    assert np.allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-6)
+    assert np.array_equal(idx, np.asarray(m.triangles)) and patch.size == len(m.triangles)
+    # outward normals of a closed box point away from its centre
+    assert np.all(((pos - pos.mean(axis=0)) * nrm).sum(axis=1) > 0)
+
+
+def test_safe_names():
+    assert T.safe_name("speed") and T.safe_name("slice_1.v-2")
+    assert not T.safe_name("") and not T.safe_name("../x") and not T.safe_name("a b") and not T.safe_name("é")
