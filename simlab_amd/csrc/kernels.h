@@ -69,6 +69,7 @@ struct DevLattice {
     double tau0, smag2, sponge_tau, sigma;
     double u_inf[3], u_init[3], grav[3], center[3];
     double sp_lo[3], sp_hi[3];
+    double sp_ilo[3], sp_ihi[3];  // 1 / width (0 where no sponge)
     int32_t abs_u, total_p, wall_model, moving, forcing, sponge;
 
     // launch shape of the stream-collide kernel

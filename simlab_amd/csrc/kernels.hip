@@ -320,15 +320,15 @@ template <class T> __device__ __forceinline__ void fneq_arr(const T p[6], T fq[1
 }
 
 template <class T> __device__ __forceinline__ T sponge_at(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z) {
-    // shader.rs:528-538
+    // shader.rs:528-538, 1 - d / w as (w - d) * (1 / w): no division per cell
     const T p[3] = {(T)x, (T)y, (T)z};
     const T hi[3] = {(T)(L.nx - 1) - p[0], (T)(L.ny - 1) - p[1], (T)(L.nz - 1) - p[2]};
     T s = (T)0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const T lo = (T)L.sp_lo[k], hw = (T)L.sp_hi[k];
-        if (lo > (T)0) s = fmax(s, (T)1 - p[k] / lo);
-        if (hw > (T)0) s = fmax(s, (T)1 - hi[k] / hw);
+        if (lo > (T)0) s = fmax(s, (lo - p[k]) * (T)L.sp_ilo[k]);
+        if (hw > (T)0) s = fmax(s, (hw - hi[k]) * (T)L.sp_ihi[k]);
     }
     return fmin(fmax(s, (T)0), (T)1);
 }
@@ -393,34 +393,38 @@ __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_
     }
     const T om = rcp_(tau);
     const T k1 = (T)1 - om;
+    // feq + (1-om) fneq, evaluated per +-pair: the even part is shared
+    const T usq = (T)1.5 * (ux * ux + uy * uy + uz * uz);
+    const T tr = (p[0] + p[1] + p[2]) * ((T)1 / (T)3);
+    f[0] = wgt<T>(0) * (m.rho1 - rho * usq) + k1 * ((T)4.5 * wgt<T>(0)) * (-tr);
+    sfor<9>([&](auto K) {
+        constexpr int i = 2 * K + 1;
+        constexpr int o = i + 1;
+        const T cu = cdot<i>(ux, uy, uz);
+        const T ev = wgt<T>(i) * (m.rho1 + rho * ((T)4.5 * cu * cu - usq)) + k1 * ((T)4.5 * wgt<T>(i)) * (cpc<i>(p) - tr);
+        const T od = ((T)3 * wgt<T>(i)) * rho * cu;
+        f[i] = ev + od;
+        f[o] = ev - od;
+    });
     if constexpr (SPONGE) {
-        T fe[19], fq[19];
-        feq_arr<T>(m.rho1, ux, uy, uz, fe);
-        fneq_arr<T>(p, fq);
-#pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = fe[i] + k1 * fq[i];
+        // absorbing layer: f += sigma sp^2 (feq(1, t) - feq(rho, u)), t the
+        // far-field velocity or the cell's own, per +-pair as above
         if (sp > (T)0 && (T)L.sigma > (T)0) {
             const T tx = L.abs_u ? (T)L.u_inf[0] : ux, ty = L.abs_u ? (T)L.u_inf[1] : uy, tz = L.abs_u ? (T)L.u_inf[2] : uz;
-            T ft[19];
-            feq_arr<T>((T)0, tx, ty, tz, ft);
+            const T tsq = (T)1.5 * (tx * tx + ty * ty + tz * tz);
             const T sg = (T)L.sigma * sp * sp;
-#pragma unroll
-            for (int i = 0; i < 19; ++i) f[i] = f[i] + sg * (ft[i] - fe[i]);
+            f[0] = f[0] + sg * (wgt<T>(0) * (rho * usq - tsq - m.rho1));
+            sfor<9>([&](auto K) {
+                constexpr int i = 2 * K + 1;
+                constexpr int o = i + 1;
+                const T cu = cdot<i>(ux, uy, uz);
+                const T ct = cdot<i>(tx, ty, tz);
+                const T ev = wgt<T>(i) * (((T)4.5 * ct * ct - tsq) - m.rho1 - rho * ((T)4.5 * cu * cu - usq));
+                const T od = ((T)3 * wgt<T>(i)) * (ct - rho * cu);
+                f[i] = f[i] + sg * (ev + od);
+                f[o] = f[o] + sg * (ev - od);
+            });
         }
-    } else {
-        // feq + (1-om) fneq, evaluated per +-pair: the even part is shared
-        const T usq = (T)1.5 * (ux * ux + uy * uy + uz * uz);
-        const T tr = (p[0] + p[1] + p[2]) * ((T)1 / (T)3);
-        f[0] = wgt<T>(0) * (m.rho1 - rho * usq) + k1 * ((T)4.5 * wgt<T>(0)) * (-tr);
-        sfor<9>([&](auto K) {
-            constexpr int i = 2 * K + 1;
-            constexpr int o = i + 1;
-            const T cu = cdot<i>(ux, uy, uz);
-            const T ev = wgt<T>(i) * (m.rho1 + rho * ((T)4.5 * cu * cu - usq)) + k1 * ((T)4.5 * wgt<T>(i)) * (cpc<i>(p) - tr);
-            const T od = ((T)3 * wgt<T>(i)) * rho * cu;
-            f[i] = ev + od;
-            f[o] = ev - od;
-        });
     }
     if constexpr (FORCE) {
         const T Gx = rho * gx, Gy = rho * gy, Gz = rho * gz;

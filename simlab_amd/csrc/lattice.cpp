@@ -180,8 +180,11 @@ const char *validate(const simlab_lattice_setup *s, std::string &msg) {
 // rows. Measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
 // searches tiles and both launches per lattice.
 constexpr uint64_t FUSED_MAX_CELLS = 16u << 20;
+
 void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override, bool fused) {
-    uint32_t bx = ((L.nx + 63) / 64) * 64;
+    // rows wider than 1024 cells split into equal pieces (a multiple of 64)
+    const uint32_t pieces = (L.nx + 1023) / 1024;
+    uint32_t bx = (((L.nx + pieces - 1) / pieces + 63) / 64) * 64;
     if (bx > 1024) bx = 1024;
     const bool half = L.storage == SIMLAB_STORAGE_FP16 || L.storage == SIMLAB_STORAGE_BF16;
     if (!std::getenv("SIMLAB_CFD_BLOCK")) {
@@ -582,6 +585,8 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         L.center[k] = su->center[k];
         L.sp_lo[k] = (double)su->sponge[2 * k];
         L.sp_hi[k] = (double)su->sponge[2 * k + 1];
+        L.sp_ilo[k] = L.sp_lo[k] > 0.0 ? 1.0 / L.sp_lo[k] : 0.0;
+        L.sp_ihi[k] = L.sp_hi[k] > 0.0 ? 1.0 / L.sp_hi[k] : 0.0;
     }
     L.abs_u = su->absorb_velocity != 0;
     L.total_p = su->total_pressure_inlet != 0;

@@ -260,9 +260,17 @@ def _bytes_per_cell(s: Storage) -> int:
     return 19 * s.pop_bytes() + 4
 
 
-def _max_cells_layout(s: Storage) -> int:
-    """tunnel.rs:140-142."""
-    return ((1 << 32) - 256) // 2 // s.pop_bytes()
+# Most cells one lattice holds whatever the memory: 32-bit cell indices with
+# signed neighbour offsets (simlab_cfd_max_cells), for every storage. The
+# reference's limit (tunnel.rs:140-142) is its GPU buffer size, half of 4 GiB
+# per population direction: 537 M cells in FP32, which on one 288 GB GPU is
+# not the binding limit, so FP32 is kept up to this one.
+MAX_CELLS_LAYOUT = (1 << 31) - 1
+
+
+def max_cells_layout(s: Storage) -> int:
+    """tunnel.rs:140-142, with this lattice's limit."""
+    return MAX_CELLS_LAYOUT
 
 
 def plan(cfg: WindConfig, m: TriMesh, unit_m: float, opt: SolverOptions, budget: Optional[int] = None) -> Plan:
@@ -368,11 +376,11 @@ def plan(cfg: WindConfig, m: TriMesh, unit_m: float, opt: SolverOptions, budget:
 
     if opt.storage is not None:
         storage = opt.storage
-    elif (bud is not None and mem_for(Storage.FP32) > bud) or n > _max_cells_layout(Storage.FP32):
+    elif (bud is not None and mem_for(Storage.FP32) > bud) or n > max_cells_layout(Storage.FP32):
         storage = Storage.FP16
     else:
         storage = Storage.FP32
-    memory = mem_for(storage) if n <= _max_cells_layout(storage) else (1 << 62)
+    memory = mem_for(storage) if n <= max_cells_layout(storage) else (1 << 62)
     frontal = math.pi * rotor_radius ** 2 if still else frontal_area(m, r)
     blockage = frontal / ((dom_hi[1] - dom_lo[1]) * (dom_hi[2] - dom_lo[2]))
 
@@ -530,6 +538,7 @@ class WindSummary:
     thin_mm: float
     cells_thin: float
     kind: str = "wind"
+    setup_s: float = 0.0        # voxelization, probes and lattice construction
 
 
 @dataclass
@@ -630,6 +639,7 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
                          total_pressure_inlet=opt.total_pressure_inlet, storage=p.storage, center=p.center,
                          wall_model=opt.wall_model)
     lat = Lattice(setup, geom, probes)
+    t_setup = time.perf_counter() - t_start
     try:
         if lat.storage != p.storage:
             warnings.append("this GPU has no half-precision storage; used FP32")
@@ -797,7 +807,8 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
             blockage=p.blockage, storage=lat.storage, memory_bytes=lat.memory_bytes,
             averaged_over=max(step - w0, 0) / unit_steps, simulated=step / unit_steps,
             time_unit="revolutions" if p.still else "passes", q_ref_pa=q_ref,
-            thin_mm=0.0 if p.spin else p.d_thin * 1e3, cells_thin=0.0 if p.spin else p.cells_thin())
+            thin_mm=0.0 if p.spin else p.d_thin * 1e3, cells_thin=0.0 if p.spin else p.cells_thin(),
+            setup_s=t_setup)
         return WindResult(summary, pres, cp, out_slices, history)
     finally:
         lat.close()

@@ -212,3 +212,13 @@ def test_result_files_round_trip(tmp_path):
 def test_safe_names():
     assert T.safe_name("speed") and T.safe_name("slice_1.v-2")
     assert not T.safe_name("") and not T.safe_name("../x") and not T.safe_name("a b") and not T.safe_name("é")
+
+
+def test_layout_limit_is_the_lattices():
+    import simlab_amd as sa
+    for st in (Storage.FP32, Storage.FP16):
+        assert T.max_cells_layout(st) == sa.max_cells(st)
+    # a sphere at 128 cells across (876 M cells) keeps FP32 without a budget
+    cfg, m = T.sphere_case(1000.0, 128)
+    p = T.plan(cfg, m, 1e-3, T.SolverOptions())
+    assert int(np.prod(p.dims)) > 537_000_000 and p.storage == Storage.FP32
