@@ -71,6 +71,21 @@ def test_sphere_re20():
     assert math.hypot(s.lift_n, s.side_n) / abs(s.drag_n) < 0.05
 
 
+@pytest.mark.parametrize("re,d,passes,tol", [(100.0, 48, 30.0, 0.03), (300.0, 48, 40.0, 0.06)])
+def test_sphere_on_48_cells(re, d, passes, tol):
+    # tunnel.rs tests: sphere_re100 and sphere_re300 (sphere_check with the
+    # reference's bounds). The reference runs these on its refined D3Q27
+    # grid; here the same plan on one uniform D3Q19 grid (46.8 M cells).
+    _say(f"study: sphere Re {re:.0f}, {d} cells across, {passes:.0f} passes")
+    cd, r = T.sphere_drag(re, d, passes)
+    s = r.summary
+    refc = T.sphere_cd_cgw(re) if re > 250.0 else 0.5 * (T.sphere_cd_cgw(re) + T.sphere_cd_sn(re))
+    _say(f"  Cd {cd:.4f} vs {refc:.4f} ({(cd / refc - 1) * 100:+.2f} %), converged {s.converged}, "
+         f"{s.simulated:.1f} passes, {s.mlups:.0f} MLUPS, {s.seconds:.1f} s")
+    assert abs(cd / refc - 1.0) < tol
+    assert math.hypot(s.lift_n, s.side_n) / abs(s.drag_n) < 0.05
+
+
 def _hover(rpm, cells, revs):
     _say(f"study: 5-inch prop hover, {rpm:+.0f} rpm, {cells} cells across, {revs} revolutions")
     r = T.prop_hover(PropSpec.five_inch(), rpm, cells, revs)
