@@ -814,6 +814,111 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         lat.close()
 
 
+# ---------------------------------------------------------------- JSON forms (serde of core config.rs and tunnel.rs)
+_STORAGE_NAME = {Storage.FP32: "Fp32", Storage.FP16: "Fp16", Storage.BF16: "Bf16", Storage.FP64: "Fp64"}
+
+
+def _vec3(v, what: str) -> tuple:
+    if not isinstance(v, (list, tuple)) or len(v) != 3:
+        raise ValueError(f"{what} must be three numbers")
+    return tuple(float(x) for x in v)
+
+
+def config_from_json(obj) -> WindConfig:
+    """A WindConfig from its JSON form (config.rs:137-188): wind_dir and
+    speed_m_s required; air, spin, quality ({"type": "draft" | "normal" |
+    "fine" | "custom", "cells_across": n}) and duration optional."""
+    if isinstance(obj, (str, bytes)):
+        import json
+        obj = json.loads(obj)
+    for k in ("wind_dir", "speed_m_s"):
+        if k not in obj:
+            raise ValueError(f"missing field `{k}`")
+    cfg = WindConfig(wind_dir=_vec3(obj["wind_dir"], "wind_dir"), speed_m_s=float(obj["speed_m_s"]))
+    air = obj.get("air")
+    if air is not None:
+        cfg.air = Air(float(air["density"]), float(air["kinematic_viscosity"]))
+    spin = obj.get("spin")
+    if spin is not None:
+        for k in ("rpm", "axis", "center_mm"):
+            if k not in spin:
+                raise ValueError(f"missing field `{k}` in spin")
+        cfg.spin = Spin(float(spin["rpm"]), _vec3(spin["axis"], "spin.axis"), _vec3(spin["center_mm"], "spin.center_mm"))
+    q = obj.get("quality")
+    if q is not None:
+        t = q.get("type")
+        if t == "custom":
+            cfg.quality = int(q["cells_across"])
+        elif t in ("draft", "normal", "fine"):
+            cfg.quality = t
+        else:
+            raise ValueError(f"unknown quality `{t}`")
+    d = obj.get("duration")
+    cfg.duration = None if d is None else float(d)
+    return cfg
+
+
+def config_to_json(cfg: WindConfig) -> dict:
+    q = {"type": "custom", "cells_across": int(cfg.quality)} if isinstance(cfg.quality, int) else {"type": cfg.quality}
+    return {"wind_dir": list(cfg.wind_dir), "speed_m_s": cfg.speed_m_s,
+            "air": {"density": cfg.air.density, "kinematic_viscosity": cfg.air.kinematic_viscosity},
+            "spin": None if cfg.spin is None else {"rpm": cfg.spin.rpm, "axis": list(cfg.spin.axis),
+                                                   "center_mm": list(cfg.spin.center_mm)},
+            "quality": q, "duration": cfg.duration}
+
+
+def _num(x):
+    if x is None:
+        return None
+    x = float(x)
+    return x if math.isfinite(x) else None
+
+
+_SUMMARY_FIELDS = ("kind", "headline", "force_n", "moment_nm", "drag_n", "lift_n", "side_n", "cd", "ref_area_mm2",
+                   "thrust_n", "torque_nm", "power_w", "reynolds", "cells", "steps", "mlups", "seconds", "converged",
+                   "warnings", "grid", "cells_across", "dx_mm", "dt_s", "mach_lattice", "tau", "blockage", "storage",
+                   "memory_bytes", "averaged_over", "simulated", "time_unit", "q_ref_pa", "thin_mm", "cells_thin")
+
+
+def summary_to_json(s: WindSummary) -> dict:
+    """The WindSummary fields in their serde order (tunnel.rs:262-304)."""
+    out = {}
+    for k in _SUMMARY_FIELDS:
+        v = getattr(s, k)
+        if k == "storage":
+            v = _STORAGE_NAME[Storage(v)]
+        elif k in ("force_n", "moment_nm"):
+            v = [_num(x) for x in v]
+        elif k == "grid":
+            v = [int(x) for x in v]
+        elif isinstance(v, (bool, str, list)) or v is None:
+            pass
+        elif k in ("cells", "steps", "cells_across", "memory_bytes"):
+            v = int(v)
+        else:
+            v = _num(v)
+        out[k] = v
+    return out
+
+
+def estimate_to_json(e: Estimate) -> dict:
+    """tunnel.rs:325-345 serde form."""
+    return {"fits": bool(e.fits), "cells": int(e.cells), "grid": [int(x) for x in e.grid],
+            "cells_across": int(e.cells_across), "memory_bytes": int(e.memory_bytes),
+            "budget_bytes": int(e.budget_bytes), "storage": _STORAGE_NAME[Storage(e.storage)], "steps": int(e.steps),
+            "seconds": float(e.seconds), "note": e.note, "thin_mm": float(e.thin_mm), "cells_thin": float(e.cells_thin),
+            "thin_warning": e.thin_warning}
+
+
+def slice_to_json(sl: FlowSlice) -> dict:
+    """FlowSlice serde form, field data included (NaN as null)."""
+    return {"name": sl.name, "label": sl.label, "origin_mm": [float(x) for x in sl.origin_mm],
+            "u_mm": [float(x) for x in sl.u_mm], "v_mm": [float(x) for x in sl.v_mm], "nu": int(sl.nu),
+            "nv": int(sl.nv),
+            "fields": [{"name": f.name, "label": f.label, "unit": f.unit, "min": _num(f.min), "max": _num(f.max),
+                        "data": [_num(x) for x in np.asarray(f.data, np.float64)]} for f in sl.fields]}
+
+
 # ---------------------------------------------------------------- result files (engine wind.rs:12-15, 440-497)
 def _range(v: np.ndarray):
     """wind.rs:452-461: finite min and max, (0, 0) when none."""

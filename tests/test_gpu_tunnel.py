@@ -144,3 +144,36 @@ def test_prop_in_forward_flight_loses_thrust():
     assert s.thrust_n < hover.thrust_n
     # the drag axis is the wind: drag is the force along -z
     assert s.drag_n == pytest.approx(-s.force_n[2])
+
+
+def test_run_stl_command_line(tmp_path):
+    # crates/cfd/examples/run_stl.rs: an STL in mm, a WindConfig, --out files
+    import json
+    from simlab_amd import run_stl
+    from simlab_amd.shapes import sphere
+    from simlab_amd.stl import write_binary_stl
+    _say("study: run_stl on a 50 mm sphere STL, 16 cells across, 3 passes")
+    stl = tmp_path / "ball.stl"
+    m = sphere([0.0, 0.0, 0.0], 25.0, 4)
+    stl.write_bytes(write_binary_stl(m))
+    out = tmp_path / "out"
+    cfg = '{"wind_dir":[1,0,0],"speed_m_s":10,"quality":{"type":"normal"},"duration":3}'
+    import contextlib
+    import io
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        assert run_stl.main([str(stl), cfg, "--cells", "16", "--out", str(out), "--history"]) == 0
+    text = buf.getvalue()
+    s = json.loads(text[:text.index("\n}") + 2])
+    _say(f"  {s['headline']}, {s['steps']} steps")
+    assert s["cells_across"] == 16 and s["grid"] == [137, 113, 113] and s["storage"] == "Fp32"
+    assert s["cd"] is not None and s["cd"] > 0 and s["thrust_n"] is None
+    assert list(s) == list(T._SUMMARY_FIELDS)
+    nv = len(m.positions)
+    assert np.fromfile(out / "pressure_pa.f32", "<f4").size == nv
+    assert np.fromfile(out / "cp.f32", "<f4").size == nv
+    js = json.loads((out / "slice.along.json").read_text())
+    assert js["nu"] * js["nv"] == len(js["fields"][0]["data"]) == np.fromfile(out / "slice.along.speed.f32", "<f4").size
+    hist = (out / "history.csv").read_text().splitlines()
+    assert hist[0] == "t,fx,fy,fz,mx,my,mz" and len(hist) > 1
+    assert "t=" in text.splitlines()[-1]
