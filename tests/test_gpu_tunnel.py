@@ -36,6 +36,11 @@ def test_sphere_re100_quick(tmp_path):
     assert s.cd == pytest.approx(cd, rel=2e-3)
     assert s.headline.startswith("Drag ") and "Cd" in s.headline
     assert s.grid == (137, 113, 113) and s.cells == 137 * 113 * 113
+    # the plan's memory estimate (the reference's formula: populations,
+    # flags, records, faces and a 64 MiB allowance) is what the lattice holds
+    cfg, m = T.sphere_case(100.0, 16)
+    est = T.plan(cfg, m, 1e-3, T.SolverOptions()).memory - (64 << 20)
+    assert abs(s.memory_bytes / est - 1.0) < 0.1
     # surface pressure: stagnation near +q at the front, finite on most vertices
     cp = r.surface_cp[np.isfinite(r.surface_cp)]
     assert cp.size > 0.9 * r.surface_cp.size
