@@ -146,6 +146,12 @@ hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 // re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)
 hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+// the three parts of launch_spin: triangle hits of a step's rotation into
+// the hit scratch, the box cells' records from them, and the clearing of
+// the entries near the part afterwards
+hipError_t launch_scatter(const DevLattice &L, const StepArgs &a, hipStream_t s);
+hipError_t launch_voxelize(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_t s);
 // resets all of the link scratch of a spinning part (setup)
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s);
 hipError_t launch_samplers(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);

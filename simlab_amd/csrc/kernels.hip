@@ -2196,8 +2196,17 @@ hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hip
 #undef BATCH
 }
 
-hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+hipError_t launch_scatter(const DevLattice &L, const StepArgs &a, hipStream_t s) {
     if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 4), 256, 0, s>>>(L, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+    k_qclear_near<<<cdiv(L.nrec, 256), 256, 0, s>>>(L, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_voxelize(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
     const uint32_t nb = cdiv(L.nrec, 256);
 #define VOX(C)                                                          \
     do {                                                                \
@@ -2211,8 +2220,14 @@ hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_
         default: VOX(CF64); break;
     }
 #undef VOX
-    k_qclear_near<<<nb, 256, 0, s>>>(L, a);
     return hipGetLastError();
+}
+
+hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    hipError_t e = launch_scatter(L, a, s);
+    if (e == hipSuccess) e = launch_voxelize(L, p, a, s);
+    if (e == hipSuccess) e = launch_qclear_near(L, a, s);
+    return e;
 }
 
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s) {

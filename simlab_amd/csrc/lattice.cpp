@@ -116,6 +116,12 @@ struct simlab_lattice {
     hipStream_t stream = nullptr;    // faces, special kernel, copies
     hipStream_t stream2 = nullptr;   // bulk kernel, concurrent with the special kernel
     hipEvent_t fork = nullptr, join = nullptr;
+    // spinning part: the next step's triangle hits are found on stream3
+    // while this step's stream-collide kernels run
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_vox = nullptr, ev_scat = nullptr;
+    bool scat_ready = false;         // the next step's hits are on stream3 behind ev_scat
+    bool spin_overlap = true;
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
     DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
@@ -140,9 +146,13 @@ struct simlab_lattice {
         if (stream) (void)hipStreamSynchronize(stream);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream2) (void)hipStreamSynchronize(stream2);
+        if (stream3) (void)hipStreamSynchronize(stream3);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (ev_vox) (void)hipEventDestroy(ev_vox);
+        if (ev_scat) (void)hipEventDestroy(ev_scat);
         if (hist_host) (void)hipHostFree(hist_host);
+        if (stream3) (void)hipStreamDestroy(stream3);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -535,6 +545,12 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     hip_check(hipStreamCreateWithFlags(&lat->stream2, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreateWithFlags(&lat->fork, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipEventCreateWithFlags(&lat->join, hipEventDisableTiming), "hipEventCreate");
+    if (geom_kind == SIMLAB_GEOM_SPIN) {
+        hip_check(hipStreamCreateWithFlags(&lat->stream3, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&lat->ev_vox, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&lat->ev_scat, hipEventDisableTiming), "hipEventCreate");
+        if (const char *e = std::getenv("SIMLAB_CFD_SPIN_OVERLAP")) lat->spin_overlap = e[0] != '0';
+    }
     for (uint64_t c = 0; c < n; ++c) lat->nbulk += (h_flags[c] == 0 || (h_flags[c] & FLAG_HALFWAY)) ? 1 : 0;
     hip_check(hipHostMalloc((void **)&lat->hist_host, (size_t)2 * KMAX * HIST_STRIDE * 8, hipHostMallocDefault), "hipHostMalloc");
     hipStream_t st = lat->stream;
@@ -668,6 +684,12 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
         idn.rot_prev[k][k] = 1.0;
     }
     hipStream_t st = lat->stream;
+    if (lat->scat_ready) {
+        // hits of a step that a failed call never took: drop them
+        hip_check(hipStreamSynchronize(lat->stream3), "GPU wait failed");
+        hip_check(launch_qclear(lat->L, st), "hit scratch clear");
+        lat->scat_ready = false;
+    }
     if (lat->timing && lat->ev.size() < 2 * KMAX + 2) {
         while (lat->ev.size() < 2 * KMAX + 2) {
             hipEvent_t e;
@@ -688,8 +710,32 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             const int par = (int)(step % 2);
             StepArgs a;
             pack_step(s, step, (uint32_t)k, a);
-            // a spinning part is re-voxelized for this step's rotation first
-            if (lat->moving) hip_check(launch_spin(lat->L, par, a, st), "re-voxelization kernels");
+            // a spinning part is re-voxelized for this step's rotation first.
+            // Its triangle hits come from stream3 when the previous step
+            // launched them there; after the voxelization, stream3 clears
+            // this step's hits and finds the next step's while this step's
+            // stream-collide kernels run (none of them reads the hit scratch)
+            if (lat->moving) {
+                if (lat->scat_ready) {
+                    hip_check(hipStreamWaitEvent(st, lat->ev_scat, 0), "event wait");
+                    lat->scat_ready = false;
+                } else {
+                    hip_check(launch_scatter(lat->L, a, st), "re-voxelization kernels");
+                }
+                hip_check(launch_voxelize(lat->L, par, a, st), "re-voxelization kernels");
+                if (lat->spin_overlap && done + k + 1 < nsteps) {
+                    StepArgs an;
+                    pack_step(still ? idn : steps[done + k + 1], step + 1, (uint32_t)(k + 1), an);
+                    hip_check(hipEventRecord(lat->ev_vox, st), "event");
+                    hip_check(hipStreamWaitEvent(lat->stream3, lat->ev_vox, 0), "event wait");
+                    hip_check(launch_qclear_near(lat->L, a, lat->stream3), "re-voxelization kernels");
+                    hip_check(launch_scatter(lat->L, an, lat->stream3), "re-voxelization kernels");
+                    hip_check(hipEventRecord(lat->ev_scat, lat->stream3), "event");
+                    lat->scat_ready = true;
+                } else {
+                    hip_check(launch_qclear_near(lat->L, a, st), "re-voxelization kernels");
+                }
+            }
             if (done + k == 0 && lat->fused_faces) {
                 // first step of this call: rebuild this step's face inputs from
                 // the stored state, then the next step's (constant where the
