@@ -1900,32 +1900,41 @@ __global__ void __launch_bounds__(256) k_sum_final(const double *partials, uint3
 
 // ---------------------------------------------------------------- probes
 // shader.rs:736-784 (run with the module of the other parity after a step)
+// Eight lanes per vertex, one per corner of the interpolation cell (the
+// eight of a vertex sit in one wave), summed with lane shuffles.
 template <class C, int P> __global__ void __launch_bounds__(256) k_sample_verts(DevLattice L, StepArgs a) {
     using T = typename C::T;
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= L.nvert) return;
-    const float4 pb = L.verts[2 * v], nn = L.verts[2 * v + 1];
-    const float bx = pb.x + L.vertex_offset * nn.x, by = pb.y + L.vertex_offset * nn.y, bz = pb.z + L.vertex_offset * nn.z;
-    // CENTER + from_body(b)
-    const float sx = (float)L.center[0] + (a.r[0][0] * bx + a.r[1][0] * by + a.r[2][0] * bz);
-    const float sy = (float)L.center[1] + (a.r[0][1] * bx + a.r[1][1] * by + a.r[2][1] * bz);
-    const float sz = (float)L.center[2] + (a.r[0][2] * bx + a.r[1][2] * by + a.r[2][2] * bz);
-    const float b0x = floorf(sx), b0y = floorf(sy), b0z = floorf(sz);
-    const float frx = sx - b0x, fry = sy - b0y, frz = sz - b0z;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = gid >> 3;
+    const int k = (int)(gid & 7u);
     float acc = 0.0f, wsum = 0.0f;
-    for (int k = 0; k < 8; ++k) {
+    if (v < L.nvert) {
+        const float4 pb = L.verts[2 * v], nn = L.verts[2 * v + 1];
+        const float bx = pb.x + L.vertex_offset * nn.x, by = pb.y + L.vertex_offset * nn.y, bz = pb.z + L.vertex_offset * nn.z;
+        // CENTER + from_body(b)
+        const float sx = (float)L.center[0] + (a.r[0][0] * bx + a.r[1][0] * by + a.r[2][0] * bz);
+        const float sy = (float)L.center[1] + (a.r[0][1] * bx + a.r[1][1] * by + a.r[2][1] * bz);
+        const float sz = (float)L.center[2] + (a.r[0][2] * bx + a.r[1][2] * by + a.r[2][2] * bz);
+        const float b0x = floorf(sx), b0y = floorf(sy), b0z = floorf(sz);
+        const float frx = sx - b0x, fry = sy - b0y, frz = sz - b0z;
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
         const int cx = (int)b0x + ox, cy = (int)b0y + oy, cz = (int)b0z + oz;
-        if (cx < 1 || cy < 1 || cz < 1 || cx >= (int)L.nx - 1 || cy >= (int)L.ny - 1 || cz >= (int)L.nz - 1) continue;
-        const float w = (ox ? frx : 1.0f - frx) * (oy ? fry : 1.0f - fry) * (oz ? frz : 1.0f - frz);
-        T rho, u[3];
-        const bool ok = cell_macro<C, P>(L, ((uint32_t)cz * L.ny + (uint32_t)cy) * L.nx + (uint32_t)cx, rho, u);
-        if (ok && w > 0.0f) {
-            acc += w * (float)rho;
-            wsum += w;
+        if (cx >= 1 && cy >= 1 && cz >= 1 && cx < (int)L.nx - 1 && cy < (int)L.ny - 1 && cz < (int)L.nz - 1) {
+            const float w = (ox ? frx : 1.0f - frx) * (oy ? fry : 1.0f - fry) * (oz ? frz : 1.0f - frz);
+            T rho, u[3];
+            const bool ok = cell_macro<C, P>(L, ((uint32_t)cz * L.ny + (uint32_t)cy) * L.nx + (uint32_t)cx, rho, u);
+            if (ok && w > 0.0f) {
+                acc = w * (float)rho;
+                wsum = w;
+            }
         }
     }
-    if (wsum > 1e-6f) {
+#pragma unroll
+    for (int m = 4; m >= 1; m >>= 1) {
+        acc += __shfl_xor(acc, m, 8);
+        wsum += __shfl_xor(wsum, m, 8);
+    }
+    if (v < L.nvert && k == 0 && wsum > 1e-6f) {
         L.vacc[2 * v] += acc / wsum;
         L.vacc[2 * v + 1] += 1.0f;
     }
@@ -2110,8 +2119,9 @@ hipError_t launch_samplers(const DevLattice &L, int p, const StepArgs &a, hipStr
 #define SAMP(C)                                                                                   \
     do {                                                                                          \
         if (L.nvert) {                                                                            \
-            if (q == 0) k_sample_verts<C, 0><<<cdiv(L.nvert, 256), 256, 0, s>>>(L, a);            \
-            else k_sample_verts<C, 1><<<cdiv(L.nvert, 256), 256, 0, s>>>(L, a);                   \
+            const uint32_t nbv = cdiv((uint64_t)L.nvert * 8, 256);                                \
+            if (q == 0) k_sample_verts<C, 0><<<nbv, 256, 0, s>>>(L, a);                           \
+            else k_sample_verts<C, 1><<<nbv, 256, 0, s>>>(L, a);                                  \
         }                                                                                         \
         if (L.npix) {                                                                             \
             if (q == 0) k_sample_slices<C, 0><<<cdiv(L.npix, 256), 256, 0, s>>>(L);               \
