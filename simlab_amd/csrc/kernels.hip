@@ -1086,8 +1086,14 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     }
 }
 
+// the sponge variant of parity 1 in 16-bit storage at 5 waves per SIMD like
+// the plain one (it needs one VGPR more than 96 left alone)
+template <class C, int P, bool FORCE, bool SPONGE> constexpr int bulk_waves() {
+    return ((std::is_same_v<C, CF16> || std::is_same_v<C, CBF16>) && P == 1 && !FORCE && SPONGE) ? 5 : 1;
+}
 template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(1024) k_bulk(DevLattice L, StepArgs a) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(bulk_waves<C, P, FORCE, SPONGE>())))
+k_bulk(DevLattice L, StepArgs a) {
     bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, blockIdx.y, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
