@@ -182,3 +182,15 @@ def test_run_stl_command_line(tmp_path):
     hist = (out / "history.csv").read_text().splitlines()
     assert hist[0] == "t,fx,fy,fz,mx,my,mz" and len(hist) > 1
     assert "t=" in text.splitlines()[-1]
+
+
+def test_validate_command_line(capsys):
+    # crates/cfd/examples/validate.rs: `validate sphere 100 16 15`
+    from simlab_amd import validate
+    _say("study: validate sphere 100 16 15")
+    assert validate.main(["sphere", "100", "16", "15"]) == 0
+    out = capsys.readouterr().out
+    line = next(l for l in out.splitlines() if l.startswith("SPHERE"))
+    assert line.startswith("SPHERE Re=100 D=16 cells: Cd=") and "grid=[137, 113, 113]" in line
+    err = float(line.split("err=")[1].split("%")[0])
+    assert abs(err) < 5.0

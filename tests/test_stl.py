@@ -123,3 +123,20 @@ def test_run_stl_estimate(tmp_path):
     r = subprocess.run([sys.executable, "-m", "simlab_amd.run_stl", str(stl), cfg, "--bogus"], cwd=ROOT,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "unknown option --bogus" in r.stderr
+
+
+def test_validate_options():
+    # crates/cfd/examples/validate.rs option words and key=value pairs
+    from simlab_amd import Storage
+    from simlab_amd.validate import options
+    o = options(["sphere", "100", "32", "20", "fp16", "side=4", "up=3", "down=6", "smag=0.2", "u=0.05", "tol=0.01",
+                 "tau=0.9", "radial=2", "axial=3", "voff=0.5", "sigma=0.2", "spin=0.5", "spout=1.5", "spside=0.6",
+                 "velsides", "absu", "p0"])
+    assert o.storage == Storage.FP16 and o.side_bc == "velocity" and o.absorb_velocity and o.total_pressure_inlet
+    assert (o.wind_margins.upstream, o.wind_margins.downstream, o.wind_margins.side) == (3.0, 6.0, 4.0)
+    assert (o.still_margins.upstream, o.still_margins.downstream, o.still_margins.side) == (3.0, 3.0, 2.0)
+    assert o.smagorinsky == 0.2 and o.max_lattice_speed == 0.05 and o.max_tip_speed == 0.05
+    assert o.tolerance == 0.01 and o.max_tau == 0.9 and o.vertex_offset == 0.5 and o.sponge_sigma == 0.2
+    assert o.sponge_widths == (0.5, 1.5, 0.6)
+    d = options(["rot", "1", "24", "4"])
+    assert d.storage is None and d.sponge_widths == (0.4, 1.0, 0.4) and d.wind_margins.side == 3.0
