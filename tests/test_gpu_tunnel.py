@@ -194,3 +194,17 @@ def test_validate_command_line(capsys):
     assert line.startswith("SPHERE Re=100 D=16 cells: Cd=") and "grid=[137, 113, 113]" in line
     err = float(line.split("err=")[1].split("%")[0])
     assert abs(err) < 5.0
+
+
+def test_sphere_study_in_fp16_storage():
+    # the same study with FP16 population storage (as a memory budget would
+    # choose it): the drag coefficient within 0.5 % of the FP32 study's
+    from simlab_amd import Storage
+    _say("study: sphere Re 100, 16 cells across, 15 passes, FP16 and FP32 storage")
+    cd16, r16 = T.sphere_drag(100.0, 16, 15.0, T.SolverOptions(storage=Storage.FP16))
+    cd32, r32 = T.sphere_drag(100.0, 16, 15.0, T.SolverOptions(storage=Storage.FP32))
+    _say(f"  Cd FP16 {cd16:.5f} ({r16.summary.simulated:.1f} passes, {r16.summary.mlups:.0f} MLUPS), "
+         f"FP32 {cd32:.5f} ({r32.summary.simulated:.1f} passes), {(cd16 / cd32 - 1) * 100:+.3f} %")
+    assert r16.summary.storage == Storage.FP16
+    assert any("FP16 storage" in w for w in r16.summary.warnings)
+    assert abs(cd16 / cd32 - 1.0) < 0.005
