@@ -42,6 +42,7 @@ struct DevLattice {
     uint32_t nx, ny, nz;
     uint32_t ncell;
     uint32_t per[3];
+    uint32_t vel_face[6];        // per face: 1 for a velocity face
     int32_t storage;             // simlab_storage
 
     void *pops;

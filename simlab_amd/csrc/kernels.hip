@@ -675,6 +675,18 @@ __device__ __forceinline__ bool is_face_cell(const DevLattice &L, uint32_t x, ui
            (!L.per[2] && (z == 0 || z == L.nz - 1));
 }
 
+// value of lane l (uniform across the wave) for every lane
+__device__ __forceinline__ int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffull), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Wave-cooperative face update after a step of parity Q: for every lane
 // that is a face source, each face cell it feeds is written by 19 lanes of
 // the wave, one direction each. All lanes of the wave must call this.
@@ -705,16 +717,18 @@ __device__ __forceinline__ void face_writes_wave(const DevLattice &L, bool src, 
     }
     const int lane = (int)__lane_id();
     while (bal) {
+        // the source lane is the same for the whole wave: read its values
+        // with v_readlane (no LDS round trip)
         const int l = __builtin_ctzll(bal);
         bal &= bal - 1;
-        const int sx = __shfl((int)x, l), sy = __shfl((int)y, l), sz = __shfl((int)z, l);
+        const int sx = read_lane((int)x, l), sy = read_lane((int)y, l), sz = read_lane((int)z, l);
         FaceSrc<T> m;
-        m.rho = __shfl(s.rho, l);
-        m.ux = __shfl(s.ux, l);
-        m.uy = __shfl(s.uy, l);
-        m.uz = __shfl(s.uz, l);
+        m.rho = read_lane(s.rho, l);
+        m.ux = read_lane(s.ux, l);
+        m.uy = read_lane(s.uy, l);
+        m.uz = read_lane(s.uz, l);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) m.p[k] = __shfl(s.p[k], l);
+        for (int k = 0; k < 6; ++k) m.p[k] = read_lane(s.p[k], l);
         const bool lo[3] = {!L.per[0] && sx == 1, !L.per[1] && sy == 1, !L.per[2] && sz == 1};
         const bool hi[3] = {!L.per[0] && sx == (int)L.nx - 2, !L.per[1] && sy == (int)L.ny - 2, !L.per[2] && sz == (int)L.nz - 2};
         for (int dz = -1; dz <= 1; ++dz) {
@@ -725,9 +739,13 @@ __device__ __forceinline__ void face_writes_wave(const DevLattice &L, bool src, 
                     if ((dx < 0 && !lo[0]) || (dx > 0 && !hi[0])) continue;
                     if (dx == 0 && dy == 0 && dz == 0) continue;
                     const int bx = sx + dx, by = sy + dy, bz = sz + dz;
-                    const uint32_t b = ((uint32_t)bz * L.ny + (uint32_t)by) * L.nx + (uint32_t)bx;
                     FaceSrc<T> fb = m;
-                    face_rule(L, L.flags[b] & 3u, fb);
+                    // the face cell's type (its flag & 3, set from its
+                    // position): velocity if any of its faces is one
+                    const bool vel = (dx < 0 && L.vel_face[0]) || (dx > 0 && L.vel_face[1]) ||
+                                     (dy < 0 && L.vel_face[2]) || (dy > 0 && L.vel_face[3]) ||
+                                     (dz < 0 && L.vel_face[4]) || (dz > 0 && L.vel_face[5]);
+                    face_rule(L, vel ? 2u : 3u, fb);
                     if (lane < 19 && face_live(L, bx, by, bz, lane))
                         face_store<C, Q>(L, bx, by, bz, lane, face_value(lane, fb), seed);
                 }

@@ -577,6 +577,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.ncell = (uint32_t)n;
     for (int a = 0; a < 3; ++a) L.per[a] = su->faces[2 * a] == SIMLAB_FACE_PERIODIC ? 1u : 0u;
+    for (int f = 0; f < 6; ++f) L.vel_face[f] = su->faces[f] == SIMLAB_FACE_VELOCITY ? 1u : 0u;
     L.storage = su->storage;
     L.pops = lat->pops.p;
     L.flags = lat->flags.as<uint32_t>();
