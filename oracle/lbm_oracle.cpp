@@ -14,9 +14,10 @@
 // (crates/cfd/tests/channel.rs, momentum.rs), closed forms (tau = 1 gives
 // feq, halfway bounce-back, Poiseuille), invariants (mass, zero force on a
 // uniform state), the committed golden vectors, and brute force on tiny
-// lattices. Functions with no such check say "parity unpinned" below: the
-// sponge and absorbing layer, the Spalding wall model, the samplers, and the
-// mean non-equilibrium stress of the spinning part's refill.
+// lattices; the absorbing layer and the samplers by the closed forms of
+// tests/test_oracle_layers_probes.py. Functions with no such check say
+// "parity unpinned" below: the Spalding wall model and the mean
+// non-equilibrium stress of the spinning part's refill.
 
 #include "../include/simlab_cfd.h"
 
@@ -350,8 +351,8 @@ template <class T> struct Lat {
         }
     }
 
-    // ---- shader.rs:528-538 ---- (parity unpinned: sponge ramp and absorbing
-    // layer are checked only against the GPU, no closed form or reference test)
+    // ---- shader.rs:528-538 ---- layer weight; with the viscosity ramp and
+    // the relaxation to feq(1, t) pinned by one- and two-step closed forms
     T sponge(uint32_t x, uint32_t y, uint32_t z) const {
         T p[3] = {(T)x, (T)y, (T)z};
         T hi[3] = {(T)(NX - 1) - p[0], (T)(NY - 1) - p[1], (T)(NZ - 1) - p[2]};
@@ -772,7 +773,8 @@ template <class T> struct Lat {
     }
 
     // ---- shader.rs:736-784 samplers (module parity p) ----
-    // (parity unpinned: the samplers are checked only against the GPU)
+    // (pinned: a linear density field at rest is read exactly, vertices at
+    // centre + body position + offset * normal, slices cell by cell)
     void sample_verts(int p, const StepU &s) {
         size_t nv = verts.size() / 8;
         V3<float> c = centerf();
