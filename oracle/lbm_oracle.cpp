@@ -15,9 +15,10 @@
 // feq, halfway bounce-back, Poiseuille), invariants (mass, zero force on a
 // uniform state), the committed golden vectors, and brute force on tiny
 // lattices; the absorbing layer and the samplers by the closed forms of
-// tests/test_oracle_layers_probes.py. Functions with no such check say
-// "parity unpinned" below: the Spalding wall model and the mean
-// non-equilibrium stress of the spinning part's refill.
+// tests/test_oracle_layers_probes.py; the Spalding wall model by
+// tests/test_oracle_wall_model.py. Functions with no such check say
+// "parity unpinned" below: the mean non-equilibrium stress of the spinning
+// part's refill.
 
 #include "../include/simlab_cfd.h"
 
@@ -369,8 +370,10 @@ template <class T> struct Lat {
         return half_to_float((uint16_t)(((i - 1) % 2) ? (w >> 16) : (w & 0xffff)));
     }
 
-    // ---- shader.rs:546-597 wall_tau (Spalding) ---- (parity unpinned: checked
-    // only against the GPU; the reference's tests do not run the wall model)
+    // ---- shader.rs:546-597 wall_tau (Spalding) ---- pinned by
+    // tests/test_oracle_wall_model.py: the wall normal and distance from the
+    // cut links, and the friction velocity against Spalding's law solved
+    // independently
     T wall_tau(const Rec<T> &r, uint32_t mask, V3<T> pc, V3<T> u, T tau_les, const V3<T> &om) const {
         V3<T> nsum = {0, 0, 0};
         for (int i = 1; i < 19; ++i)
@@ -1257,6 +1260,22 @@ void oracle_collide_f64(const double *fin, double tau0, double cs, const double 
     StepU s; std::memset(&s, 0, sizeof s);
     L.main_cell(0, 0, s, false);
     for (int i = 0; i < 19; ++i) fout[i] = L.pd[opp(i)];   // store_out(0) writes (n, opp i)
+}
+
+// The Spalding wall model of one wall cell (lattice units): its link mask
+// and q per direction (as the record packs them, f16), the cell position
+// relative to the spin centre, its velocity, the LES tau, the angular
+// velocity and tau0; returns the cell's tau, for the wall-model tests.
+double oracle_wall_tau_f64(uint32_t mask, const float *q19, const double pc[3], const double u[3], double tau_les,
+                           const double om[3], double tau0) {
+    Lat<double> L;
+    L.tau0 = tau0;
+    Rec<double> r;
+    r.mask = mask;
+    for (int k = 0; k < 9; ++k)
+        r.q[k] = (uint32_t)f16_bits_host(q19[2 * k + 1]) | ((uint32_t)f16_bits_host(q19[2 * k + 2]) << 16);
+    return L.wall_tau(r, mask, V3<double>{pc[0], pc[1], pc[2]}, V3<double>{u[0], u[1], u[2]}, tau_les,
+                      V3<double>{om[0], om[1], om[2]});
 }
 
 }  // extern "C"

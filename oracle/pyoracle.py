@@ -69,6 +69,8 @@ def lib():
         L.oracle_aa_maps.argtypes = [P, C.c_int, P, P]
         L.oracle_collide_f64.argtypes = [P, C.c_double, C.c_double, P, P]
         L.oracle_set_fused.argtypes = [V, C.c_int]
+        L.oracle_wall_tau_f64.restype = C.c_double
+        L.oracle_wall_tau_f64.argtypes = [C.c_uint32, P, P, P, C.c_double, P, C.c_double]
         _lib = L
     return _lib
 
@@ -238,3 +240,11 @@ def collide_f64(fin, tau0, cs=0.0, grav=(0.0, 0.0, 0.0)):
     out = np.zeros(19, np.float64)
     lib().oracle_collide_f64(_ptr(fin), float(tau0), float(cs), _ptr(g), _ptr(out))
     return out
+
+
+def wall_tau_f64(mask, q19, pc, u, tau_les, om, tau0):
+    """The oracle's Spalding wall model for one wall cell."""
+    q = np.ascontiguousarray(q19, np.float32)
+    a = [np.ascontiguousarray(v, np.float64) for v in (pc, u, om)]
+    return float(lib().oracle_wall_tau_f64(int(mask), _ptr(q), _ptr(a[0]), _ptr(a[1]), float(tau_les), _ptr(a[2]),
+                                           float(tau0)))
