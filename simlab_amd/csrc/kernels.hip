@@ -1090,16 +1090,18 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     // solid, face and wall-record cells are not
     const bool mine = may && (fg & 3u) == 0 && rec_of(fg) == 0;
     const uint32_t hwm = (fg & FLAG_HALFWAY) ? ((fg >> 1) & 0x7FFFEu) : 0u;
+    T rho, u[3], pp[6];
     if (mine) {
         if (hwm) halfway_links<C, P>(L, a, nb, hwm, f);
-        T rho, u[3], pp[6];
         collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
         scatter_post<C, P, false>(L, nb, f, a.seed);
     }
     const bool src = mine && is_face_source(L, x, y, z);
     if (__ballot(src)) {
+        // the source's post-collision state from the collision's own
+        // rho, u and stress (the moments of f where the layer acts)
         FaceSrc<T> s;
-        if (src) s = face_source(L, f);
+        if (src) s = post_state<T, SPONGE>(L, f, rho, u, pp, src);
         face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
     }
 }
