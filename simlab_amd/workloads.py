@@ -245,3 +245,39 @@ def spinning_prop(dims=(384, 384, 384), radius: float = 120.0, tip: float = 0.08
     meta = dict(nu=nu, tau0=0.5 + 3.0 * nu, omega=omega, radius=radius, tip=tip, u_axial=u_axial, seed=seed,
                 ntri=int(len(vox.tris)), box_lo=box_lo, box_n=box_n, occ_h=occ_h, axis=(0.0, 0.0, 1.0))
     return setup, Geometry.spin(vox, box_lo, box_n), meta
+
+
+def orbit_sphere(r: float, ro: float, u: float, re: float, fixed: bool = False, spin_path: bool = False,
+                 storage: Storage = Storage.FP32):
+    """crates/cfd/examples/orbit_sphere.rs: a sphere of radius r orbiting
+    about +z at radius ro with speed u through still fluid (open faces, a
+    10-cell absorbing layer all round), or, with `fixed`, the same sphere
+    held in a uniform stream u (velocity inlet at -x) in a domain of the
+    same size, voxelized statically or, with `spin_path`, through the
+    spinning-part path at rest. Returns (setup, geometry, meta) with meta
+    holding the Stokes drag 3 pi nu d u and, for the orbit, omega = u / ro."""
+    from . import voxelize_spin
+    nu = u * 2.0 * r / re
+    nxy, nz = int(2.0 * (ro + r) + 40.0), int(2.0 * r + 40.0)
+    c = (nxy / 2.0, nxy / 2.0, nz / 2.0)
+    kw = dict(nu=nu, smagorinsky=0.0, sponge=(10,) * 6, sponge_tau=1.0, sponge_sigma=0.1, storage=storage)
+    meta = dict(nu=nu, stokes=3.0 * math.pi * nu * 2.0 * r * u, omega=u / ro)
+    if fixed:
+        dims = (nxy, int(2.0 * r + 40.0) * 2, nz)
+        cc = (dims[0] / 2.0, dims[1] / 2.0, nz / 2.0)
+        if spin_path:
+            m = sphere([0.0, 0.0, 0.0], r, 5)
+            vox = voxelize_spin(m.positions, m.triangles, (0.0, 0.0, 1.0), 0.125)
+            b = int(r) + 4
+            geom = Geometry.spin(vox, tuple(int(v) - b for v in cc), (2 * b + 1,) * 3)
+        else:
+            geom = static_geometry(sphere(list(cc), r, 5), dims)
+        setup = LatticeSetup(dims=dims, faces=(FaceBc.VELOCITY,) + (FaceBc.OPEN,) * 5, u_inf=(u, 0.0, 0.0),
+                             u_init=(u, 0.0, 0.0), total_pressure_inlet=True, center=cc, **kw)
+        return setup, geom, meta
+    m = sphere([ro, 0.0, 0.0], r, 5)
+    vox = voxelize_spin(m.positions, m.triangles, (0.0, 0.0, 1.0), 0.125)
+    b = int(math.ceil(ro + r + 3.0))
+    geom = Geometry.spin(vox, (int(c[0]) - b, int(c[1]) - b, int(c[2]) - int(r) - 3), (2 * b + 1, 2 * b + 1, int(2 * r) + 7))
+    setup = LatticeSetup(dims=(nxy, nxy, nz), faces=(FaceBc.OPEN,) * 6, center=c, **kw)
+    return setup, geom, meta
