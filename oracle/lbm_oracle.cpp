@@ -16,9 +16,8 @@
 // uniform state), the committed golden vectors, and brute force on tiny
 // lattices; the absorbing layer and the samplers by the closed forms of
 // tests/test_oracle_layers_probes.py; the Spalding wall model by
-// tests/test_oracle_wall_model.py. Functions with no such check say
-// "parity unpinned" below: the mean non-equilibrium stress of the spinning
-// part's refill.
+// tests/test_oracle_wall_model.py; the refill's mean stress by
+// tests/test_oracle_spin.py.
 
 #include "../include/simlab_cfd.h"
 
@@ -311,6 +310,25 @@ template <class T> struct Lat {
     }
 
     // ---- shader.rs:341-396 ----
+    // the refill's share of one fluid neighbour (shader.rs:1062-1078): the
+    // traceless non-equilibrium stress of its recorded post-collision state
+    static void add_neighbour_stress(const Rec<T> &R2, T pis[6]) {
+        T fp[19];
+        fp[0] = 0;
+        for (int j = 1; j < 19; ++j) fp[j] = R2.fpost[j - 1];
+        Mom<T> mo2 = moments(fp);
+        T r2 = R2.mac[0];
+        V3<T> u2 = {R2.mac[1], R2.mac[2], R2.mac[3]};
+        pis[3] = pis[3] + (mo2.s[3] - r2 * u2.x * u2.y);
+        pis[4] = pis[4] + (mo2.s[4] - r2 * u2.x * u2.z);
+        pis[5] = pis[5] + (mo2.s[5] - r2 * u2.y * u2.z);
+        T tr = (mo2.s[0] + mo2.s[1] + mo2.s[2]) / (T)3;
+        T uu = dot(u2, u2);
+        pis[0] = pis[0] + (mo2.s[0] - tr - r2 * (u2.x * u2.x - uu / (T)3));
+        pis[1] = pis[1] + (mo2.s[1] - tr - r2 * (u2.y * u2.y - uu / (T)3));
+        pis[2] = pis[2] + (mo2.s[2] - tr - r2 * (u2.z * u2.z - uu / (T)3));
+    }
+
     static Mom<T> moments(const T f[19]) {
         Mom<T> m;
         T r = 0, jx = 0, jy = 0, jz = 0, s[6] = {0, 0, 0, 0, 0, 0};
@@ -733,26 +751,12 @@ template <class T> struct Lat {
                         const Rec<T> &R2 = recs[k2];
                         rs = rs + R2.mac[0];
                         cnt = cnt + 1;
-                        T fp[19];
-                        fp[0] = 0;
-                        for (int j = 1; j < 19; ++j) fp[j] = R2.fpost[j - 1];
-                        Mom<T> mo2 = moments(fp);
-                        T r2 = R2.mac[0];
-                        V3<T> u2 = {R2.mac[1], R2.mac[2], R2.mac[3]};
-                        pis[3] = pis[3] + (mo2.s[3] - r2 * u2.x * u2.y);
-                        pis[4] = pis[4] + (mo2.s[4] - r2 * u2.x * u2.z);
-                        pis[5] = pis[5] + (mo2.s[5] - r2 * u2.y * u2.z);
-                        T tr = (mo2.s[0] + mo2.s[1] + mo2.s[2]) / (T)3;
-                        T uu = dot(u2, u2);
-                        pis[0] = pis[0] + (mo2.s[0] - tr - r2 * (u2.x * u2.x - uu / (T)3));
-                        pis[1] = pis[1] + (mo2.s[1] - tr - r2 * (u2.y * u2.y - uu / (T)3));
-                        pis[2] = pis[2] + (mo2.s[2] - tr - r2 * (u2.z * u2.z - uu / (T)3));
+                        add_neighbour_stress(R2, pis);
                     }
                 }
             }
-            // (the mean non-equilibrium stress below is parity unpinned: zero in
-            // the refill test's equilibrium flow, checked otherwise only
-            // against the GPU)
+            // (the mean non-equilibrium stress: add_neighbour_stress, pinned
+            // by tests/test_oracle_spin.py)
             T rho = cnt > 0 ? rs / cnt : (T)1;
             V3<float> uwf = cross(V3<float>{s.omega[0], s.omega[1], s.omega[2]}, pc);
             V3<T> uw = {(T)uwf.x, (T)uwf.y, (T)uwf.z};
@@ -1276,6 +1280,19 @@ double oracle_wall_tau_f64(uint32_t mask, const float *q19, const double pc[3], 
         r.q[k] = (uint32_t)f16_bits_host(q19[2 * k + 1]) | ((uint32_t)f16_bits_host(q19[2 * k + 2]) << 16);
     return L.wall_tau(r, mask, V3<double>{pc[0], pc[1], pc[2]}, V3<double>{u[0], u[1], u[2]}, tau_les,
                       V3<double>{om[0], om[1], om[2]});
+}
+
+// The refill's mean non-equilibrium stress over n neighbour records
+// (post-collision f~ of directions 1..18, then rho, u), for its tests.
+void oracle_refill_stress_f64(const double *fpost18, const double *mac4, int n, double pis_out[6]) {
+    double pis[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        Rec<double> R;
+        for (int j = 0; j < 18; ++j) R.fpost[j] = fpost18[18 * k + j];
+        for (int j = 0; j < 4; ++j) R.mac[j] = mac4[4 * k + j];
+        Lat<double>::add_neighbour_stress(R, pis);
+    }
+    for (int j = 0; j < 6; ++j) pis_out[j] = n > 0 ? pis[j] / n : 0.0;
 }
 
 }  // extern "C"

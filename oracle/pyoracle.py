@@ -71,6 +71,7 @@ def lib():
         L.oracle_set_fused.argtypes = [V, C.c_int]
         L.oracle_wall_tau_f64.restype = C.c_double
         L.oracle_wall_tau_f64.argtypes = [C.c_uint32, P, P, P, C.c_double, P, C.c_double]
+        L.oracle_refill_stress_f64.argtypes = [P, P, C.c_int, P]
         _lib = L
     return _lib
 
@@ -248,3 +249,13 @@ def wall_tau_f64(mask, q19, pc, u, tau_les, om, tau0):
     a = [np.ascontiguousarray(v, np.float64) for v in (pc, u, om)]
     return float(lib().oracle_wall_tau_f64(int(mask), _ptr(q), _ptr(a[0]), _ptr(a[1]), float(tau_les), _ptr(a[2]),
                                            float(tau0)))
+
+
+def refill_stress_f64(fpost18, mac4):
+    """The oracle's mean non-equilibrium stress of a refill over neighbour
+    records (n x 18 post-collision f~, n x 4 rho and u): xx yy zz xy xz yz."""
+    f = np.ascontiguousarray(fpost18, np.float64).reshape(-1, 18)
+    m = np.ascontiguousarray(mac4, np.float64).reshape(-1, 4)
+    out = np.zeros(6)
+    lib().oracle_refill_stress_f64(_ptr(f), _ptr(m), len(f), _ptr(out))
+    return out

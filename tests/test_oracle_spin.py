@@ -113,3 +113,34 @@ def test_refill_and_momentum_of_uncovered_and_covered_cells():
         feq = WQ * (3.0 * cu + 4.5 * cu * cu - 1.5 * uw @ uw)
         got = pops[st[cell].astype(np.int64)]
         assert np.max(np.abs(got - feq)) < 1e-12, (cell, got - feq)
+
+
+def test_refill_mean_stress_is_the_neighbours_traceless_stress():
+    # shader.rs:1062-1082: a refilled cell takes the mean non-equilibrium
+    # stress of its fluid neighbours' records. A record holding
+    # feq(rho, u) + fneq(Pi) (the second moment of the D3Q19 equilibrium is
+    # (rho - 1)/3 I + rho u u in deviations, that of fneq(Pi) is Pi) gives back
+    # Pi less a third of its trace on the diagonal, whatever rho and u; the
+    # rest population, which records do not keep, does not enter.
+    rng = np.random.default_rng(7)
+    n = 5
+    fpost, mac, want = [], [], np.zeros(6)
+    for _ in range(n):
+        rho = 1.0 + 0.02 * rng.standard_normal()
+        u = 0.05 * rng.standard_normal(3)
+        P = 1e-3 * rng.standard_normal((3, 3))
+        P = P + P.T
+        cu = CV @ u
+        f = WQ * ((rho - 1.0) + rho * (3.0 * cu + 4.5 * cu * cu - 1.5 * u @ u))
+        f = f + 4.5 * WQ * (np.einsum("ia,ib,ab->i", CV, CV, P) - np.trace(P) / 3.0)
+        fpost.append(f[1:])
+        mac.append([rho, *u])
+        D = P - np.trace(P) / 3.0 * np.eye(3)
+        want += np.array([D[0, 0], D[1, 1], D[2, 2], D[0, 1], D[0, 2], D[1, 2]]) / n
+    got = O.refill_stress_f64(np.array(fpost), np.array(mac))
+    assert np.allclose(got, want, rtol=1e-10, atol=1e-15)
+    # one neighbour at equilibrium: no stress
+    rho, u = 1.01, np.array([0.03, -0.01, 0.02])
+    cu = CV @ u
+    f = WQ * ((rho - 1.0) + rho * (3.0 * cu + 4.5 * cu * cu - 1.5 * u @ u))
+    assert np.allclose(O.refill_stress_f64(f[None, 1:], np.array([[rho, *u]])), 0.0, atol=1e-17)
