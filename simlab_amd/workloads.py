@@ -281,3 +281,46 @@ def orbit_sphere(r: float, ro: float, u: float, re: float, fixed: bool = False, 
     geom = Geometry.spin(vox, (int(c[0]) - b, int(c[1]) - b, int(c[2]) - int(r) - 3), (2 * b + 1, 2 * b + 1, int(2 * r) + 7))
     setup = LatticeSetup(dims=(nxy, nxy, nz), faces=(FaceBc.OPEN,) * 6, center=c, **kw)
     return setup, geom, meta
+
+
+def naca0012(c: float, alpha_deg: float, re: float, smagorinsky: float = 0.1, storage: Storage = Storage.FP32):
+    """crates/cfd/examples/airfoil.rs: a NACA 0012 section of chord c cells,
+    extruded through a 4-cell periodic span, at angle of attack alpha (about
+    the quarter chord, nose up for alpha > 0) in a 12c x 8c tunnel: velocity
+    inlet 0.075 at -x, open faces in x and y with absorbing layers, the
+    quarter chord at (4c, 4c). Lift is F_y, drag F_x; meta['q'] is the
+    dynamic pressure times chord times span, per step."""
+    import numpy as np
+    from .shapes import TriMesh, fix_orientation
+    u = 0.075
+    nu = u * c / re
+    nx, ny, nz = int(12.0 * c), int(8.0 * c), 4
+    x0, y0 = 4.0 * c, 4.0 * c
+    a = math.radians(alpha_deg)
+    nsec = 200
+    sec = []
+    for i in range(nsec):
+        th = 2.0 * math.pi * i / nsec
+        xc = 0.5 * (1.0 + math.cos(th))
+        t = 0.6 * (0.2969 * math.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2 + 0.2843 * xc ** 3 - 0.1036 * xc ** 4)
+        yc = t if th < math.pi else -t
+        px, py = (xc - 0.25) * c, yc * c
+        sec.append([x0 + px * math.cos(a) + py * math.sin(a), y0 - px * math.sin(a) + py * math.cos(a)])
+    verts = [[p[0], p[1], z] for z in (-3.0, nz + 3.0) for p in sec]
+    n = nsec
+    tris = []
+    for i in range(n):
+        j = (i + 1) % n
+        tris += [[i, j, n + j], [i, n + j, n + i]]
+    for i in range(1, n - 1):
+        tris += [[0, i + 1, i], [n, n + i, n + i + 1]]
+    m = TriMesh(verts, tris)
+    fix_orientation(m)
+    dims = (nx, ny, nz)
+    sp = int(0.5 * c)
+    setup = LatticeSetup(dims=dims, faces=(FaceBc.VELOCITY, FaceBc.OPEN, FaceBc.OPEN, FaceBc.OPEN, FaceBc.PERIODIC,
+                                           FaceBc.PERIODIC),
+                         nu=nu, smagorinsky=smagorinsky, u_inf=(u, 0.0, 0.0), u_init=(u, 0.0, 0.0),
+                         sponge=(sp, 2 * sp, sp, sp, 0, 0), sponge_tau=1.0, sponge_sigma=0.1,
+                         total_pressure_inlet=True, storage=storage, center=(x0, y0, 0.0))
+    return setup, static_geometry(m, dims), dict(u=u, nu=nu, q=0.5 * u * u * c * nz, steps_per_pass=c / u)
