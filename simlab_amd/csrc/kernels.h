@@ -65,6 +65,10 @@ struct DevLattice {
     uint32_t *hw_cells;          // [nhw], sorted
     uint32_t hw_blocks;          // k_special blocks for them
     double *partials;            // [KMAX][wall_blocks + hw_blocks][8] per-block F, T of each step of a chunk
+    // x-face sources that feed only their x-face cell leave their post-collision
+    // state here ([2][nz][ny] entries of 12 floats) for k_xfaces after the step
+    float *xsrc;
+    uint32_t xdefer;
 
     // physics (lattice units)
     double tau0, smag2, sponge_tau, sigma;
@@ -139,6 +143,8 @@ hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
 // standalone face update before a step of this parity (seed of that step)
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
+// the x-face cells from the sources' states the step left in xsrc
+hipError_t launch_xfaces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
 // k_special on s_special, k_bulk on s_bulk (independent within a step)
 hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 // the same as one kernel on one stream (the tile's bx * by must divide SPECIAL_BLOCK)

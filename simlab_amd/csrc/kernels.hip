@@ -1102,8 +1102,50 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
         // rho, u and stress (the moments of f where the layer acts)
         FaceSrc<T> s;
         if (src) s = post_state<T, SPONGE>(L, f, rho, u, pp, src);
-        face_writes_wave<C, P>(L, src, x, y, z, s, a.seed + 2654435761u);
+        // a source at x = 1 or nx - 2 that feeds no y or z face cell leaves
+        // its state for k_xfaces instead of holding its wave for one lane
+        bool xs = false;
+        if constexpr (std::is_same_v<T, float>) {
+            xs = src && L.xdefer && (x == 1 || x == L.nx - 2) && (L.per[1] || (y != 1 && y != L.ny - 2)) &&
+                 (L.per[2] || (z != 1 && z != L.nz - 2));
+            if (xs) {
+                float4 *o = (float4 *)(L.xsrc + ((size_t)((x == 1 ? 0u : 1u) * L.nz + z) * L.ny + y) * 12);
+                o[0] = make_float4(s.rho, s.ux, s.uy, s.uz);
+                o[1] = make_float4(s.p[0], s.p[1], s.p[2], s.p[3]);
+                o[2] = make_float4(s.p[4], s.p[5], 0.0f, 0.0f);
+            }
+        }
+        const bool coop = src && !xs;
+        if (__ballot(coop)) face_writes_wave<C, P>(L, coop, x, y, z, s, a.seed + 2654435761u);
     }
+}
+
+// The x-face cells of the sources that left their state in xsrc: one thread
+// per face cell and direction, the writes face_writes_wave would have done
+// (same rule, values, slots and dither).
+template <class C, int Q> __global__ void __launch_bounds__(256) k_xfaces(DevLattice L, uint32_t seed) {
+    using T = typename C::T;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ne = 2ull * L.nz * L.ny;
+    if (g >= ne * 19) return;
+    const uint32_t e = (uint32_t)(g / 19), i = (uint32_t)(g - (uint64_t)e * 19);
+    const uint32_t side = e / (L.nz * L.ny), r = e - side * L.nz * L.ny;
+    const uint32_t z = r / L.ny, y = r - z * L.ny;
+    const uint32_t sx = side ? L.nx - 2 : 1u;
+    // the source must be a colliding x-only source of the bulk kernel
+    if (!(L.per[1] || (y >= 1 && y <= L.ny - 2 && y != 1 && y != L.ny - 2))) return;
+    if (!(L.per[2] || (z >= 1 && z <= L.nz - 2 && z != 1 && z != L.nz - 2))) return;
+    const uint32_t fg = L.flags[((size_t)z * L.ny + y) * L.nx + sx];
+    if ((fg & 3u) != 0 || rec_of(fg) != 0) return;
+    const int bx = side ? (int)L.nx - 1 : 0;
+    if (!face_live(L, bx, (int)y, (int)z, (int)i)) return;
+    const float4 *o = (const float4 *)(L.xsrc + (size_t)e * 12);
+    const float4 a0 = o[0], a1 = o[1], a2 = o[2];
+    FaceSrc<T> fb;
+    fb.rho = a0.x; fb.ux = a0.y; fb.uy = a0.z; fb.uz = a0.w;
+    fb.p[0] = a1.x; fb.p[1] = a1.y; fb.p[2] = a1.z; fb.p[3] = a1.w; fb.p[4] = a2.x; fb.p[5] = a2.y;
+    face_rule(L, L.vel_face[side] ? 2u : 3u, fb);
+    face_store<C, Q>(L, bx, (int)y, (int)z, (int)i, face_value((int)i, fb), seed);
 }
 
 // the sponge variant of parity 1 in 16-bit storage at 5 waves per SIMD like
@@ -2071,6 +2113,24 @@ hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s
         default: FACES(CF64); break;
     }
 #undef FACES
+    return hipGetLastError();
+}
+
+hipError_t launch_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
+    if (!L.xdefer) return hipSuccess;
+    const uint32_t nb = cdiv(2ull * L.nz * L.ny * 19, 256);
+#define XF(C)                                                       \
+    do {                                                            \
+        if (p == 0) k_xfaces<C, 0><<<nb, 256, 0, s>>>(L, seed);     \
+        else k_xfaces<C, 1><<<nb, 256, 0, s>>>(L, seed);            \
+    } while (0)
+    switch (L.storage) {
+        case SIMLAB_STORAGE_FP32: XF(CF32); break;
+        case SIMLAB_STORAGE_FP16: XF(CF16); break;
+        case SIMLAB_STORAGE_BF16: XF(CBF16); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef XF
     return hipGetLastError();
 }
 

@@ -124,6 +124,7 @@ struct simlab_lattice {
     bool spin_overlap = true;
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
+    DevBuf xsrc;                     // deferred x-face sources (L.xsrc)
     DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
         counter, dump, mass_part, mass_out, occ, tris, qscr, near, verts, vacc, slicedef, sacc;
     bool fused_faces = true;   // face update fused into the previous step
@@ -666,6 +667,17 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     if (const char *e = std::getenv("SIMLAB_CFD_SERIAL")) lat->concurrent = !(e[0] == '1');
     lat->partials.alloc((size_t)KMAX * (L.wall_blocks + L.hw_blocks) * 8 * 8, mem);
     L.partials = lat->partials.as<double>();
+    // x-face cells written by k_xfaces after each step (16-bit storage: the
+    // bulk kernel 9 % faster; in FP32 it measured slower)
+    L.xdefer = (!L.per[0] && L.nx >= 4 &&
+                (su->storage == SIMLAB_STORAGE_FP16 || su->storage == SIMLAB_STORAGE_BF16)) ? 1u : 0u;
+    if (const char *e = std::getenv("SIMLAB_CFD_XDEFER")) L.xdefer = L.xdefer && e[0] != '0';
+    if (L.xdefer) {
+        lat->xsrc.alloc((size_t)2 * L.nz * L.ny * 12 * sizeof(float), mem);
+        L.xsrc = lat->xsrc.as<float>();
+    } else {
+        L.xsrc = nullptr;
+    }
     lat->memory_bytes = mem;
 
     hip_check(launch_init(L, st), "init kernel");
@@ -772,6 +784,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                 hip_check(launch_stream_collide(lat->L, par, a, st, st), "stream-collide kernels");
                 if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
             }
+            if (lat->L.xdefer) hip_check(launch_xfaces(lat->L, par, a.seed + 2654435761u, st), "x-face kernel");
             if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
         }
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
@@ -1381,6 +1394,7 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
             for (size_t i = 0; i < n; ++i) {
                 hip_check(hipStreamSynchronize(ml[i]->stream), "GPU wait failed");
                 hl[i] = ml[i]->L;
+                hl[i].xdefer = 0;   // the batch kernels write every face cell themselves
                 // the batch's blocks have SPECIAL_BLOCK threads: a tile that
                 // does not divide that is replaced (in the batch's copy):
                 // 128 x 1 for 16-bit storage, else whole rows of up to 256
