@@ -667,9 +667,11 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     if (const char *e = std::getenv("SIMLAB_CFD_SERIAL")) lat->concurrent = !(e[0] == '1');
     lat->partials.alloc((size_t)KMAX * (L.wall_blocks + L.hw_blocks) * 8 * 8, mem);
     L.partials = lat->partials.as<double>();
-    // x-face cells written by k_xfaces after each step (16-bit storage: the
-    // bulk kernel 9 % faster; in FP32 it measured slower)
-    L.xdefer = (!L.per[0] && L.nx >= 4 &&
+    // x-face cells written by k_xfaces after each step: 16-bit storage and
+    // lattices of up to FUSED_MAX_CELLS (the one-kernel step 9 % faster, the
+    // whole step 4 %); in FP32, and on larger lattices, whose face rows have
+    // left the caches by then (384^3: 10 % slower), it measured slower
+    L.xdefer = (!L.per[0] && L.nx >= 4 && n <= FUSED_MAX_CELLS &&
                 (su->storage == SIMLAB_STORAGE_FP16 || su->storage == SIMLAB_STORAGE_BF16)) ? 1u : 0u;
     if (const char *e = std::getenv("SIMLAB_CFD_XDEFER")) L.xdefer = L.xdefer && e[0] != '0';
     if (L.xdefer) {
