@@ -1121,17 +1121,20 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
 }
 
 // The x-face cells of the sources that left their state in xsrc: one thread
-// per face cell and direction, the writes face_writes_wave would have done
+// per face cell and direction into the domain (the five with c_x = +1 on the
+// -x face, -1 on the +x face), the writes face_writes_wave would have done
 // (same rule, values, slots and dither).
 template <class C, int Q> __global__ void __launch_bounds__(256) k_xfaces(DevLattice L, uint32_t seed) {
     using T = typename C::T;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t ne = 2ull * L.nz * L.ny;
-    if (g >= ne * 19) return;
-    const uint32_t e = (uint32_t)(g / 19), i = (uint32_t)(g - (uint64_t)e * 19);
+    if (g >= ne * 5) return;
+    const uint32_t e = (uint32_t)(g / 5), k = (uint32_t)(g - (uint64_t)e * 5);
     const uint32_t side = e / (L.nz * L.ny), r = e - side * L.nz * L.ny;
     const uint32_t z = r / L.ny, y = r - z * L.ny;
     const uint32_t sx = side ? L.nx - 2 : 1u;
+    // directions 1, 7, 9, 13, 15 have c_x = +1, their opposites 2, 8, 10, 14, 16 c_x = -1
+    const uint32_t i = (k == 0 ? 1u : 5u + 2u * k + (k > 2 ? 2u : 0u)) + side;
     // the source must be a colliding x-only source of the bulk kernel
     if (!(L.per[1] || (y >= 1 && y <= L.ny - 2 && y != 1 && y != L.ny - 2))) return;
     if (!(L.per[2] || (z >= 1 && z <= L.nz - 2 && z != 1 && z != L.nz - 2))) return;
@@ -2118,7 +2121,7 @@ hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s
 
 hipError_t launch_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
     if (!L.xdefer) return hipSuccess;
-    const uint32_t nb = cdiv(2ull * L.nz * L.ny * 19, 256);
+    const uint32_t nb = cdiv(2ull * L.nz * L.ny * 5, 256);
 #define XF(C)                                                       \
     do {                                                            \
         if (p == 0) k_xfaces<C, 0><<<nb, 256, 0, s>>>(L, seed);     \
