@@ -1,0 +1,47 @@
+"""The environment switches change how a step is launched, not what it
+computes: the spinning part's triangle scatter beside the step or before
+it, and the one-kernel step against the split one."""
+import numpy as np
+import pytest
+
+import simlab_amd as sa
+from simlab_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _prop_run(monkeypatch, overlap):
+    monkeypatch.setenv("SIMLAB_CFD_SPIN_OVERLAP", "1" if overlap else "0")
+    setup, geom, meta = W.spinning_prop(dims=(80, 80, 48), radius=24.0, storage=sa.Storage.FP32)
+    lat = sa.Lattice(setup, geom)
+    drv = W.SpinDriver(meta["axis"], meta["omega"])
+    h = np.concatenate([lat.run(drv.next(13)), lat.run(drv.next(20))])
+    return h, lat.get_pops()
+
+
+def test_spin_scatter_overlap_changes_nothing(monkeypatch):
+    print("case spin overlap on/off", flush=True)
+    h1, p1 = _prop_run(monkeypatch, True)
+    h0, p0 = _prop_run(monkeypatch, False)
+    # the same kernels on the same data, only the stream each runs on differs
+    assert np.array_equal(h1, h0) and np.array_equal(p1, p0)
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP32, sa.Storage.FP16])
+def test_split_and_one_kernel_steps_agree(monkeypatch, storage):
+    print(f"case split vs one-kernel step {storage.name}", flush=True)
+    out = {}
+    for mode in ("fused", "split"):
+        monkeypatch.setenv("SIMLAB_CFD_LAUNCH", mode)
+        monkeypatch.setenv("SIMLAB_CFD_XDEFER", "0")
+        setup, geom, meta = W.sphere_channel((96, 48, 40), 6.0, 30.0, 200.0, 0.05, 0.1, storage, subdiv=3)
+        lat = sa.Lattice(setup, geom)
+        out[mode] = (lat.run_still(30), lat.get_pops())
+    (hf, pf), (hs, ps) = out["fused"], out["split"]
+    # separately compiled kernels: a multiply-add may round differently
+    assert np.allclose(hf, hs, rtol=1e-3, atol=1e-3 * np.abs(hf).max())
+    if storage == sa.Storage.FP32:
+        assert np.max(np.abs(pf.astype(np.float64) - ps.astype(np.float64))) < 1e-5
+    else:
+        d = np.abs(pf.view(np.float16).astype(np.float64) - ps.view(np.float16).astype(np.float64)).max() / 32768.0
+        assert d < 1e-4
