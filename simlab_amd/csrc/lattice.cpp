@@ -1104,19 +1104,23 @@ struct Tuned {
 };
 std::map<TuneKey, Tuned> g_tune;   // best tile and launch per lattice size
 
-// tiles to try: x extents of a whole row rounded to a wave and its halves
-// (>= 64), rows per tile so a tile has 64 .. 1024 threads
+// tiles to try: the lattice's current tile first, then x extents of a whole
+// row rounded to a wave, its whole-wave divisors (384: 192, 128, 64) and 64,
+// 128, 256, with rows per tile so a tile has 64 .. 1024 threads
 std::vector<std::pair<uint32_t, uint32_t>> tile_candidates(const DevLattice &L) {
-    std::vector<std::pair<uint32_t, uint32_t>> out;
-    uint32_t row = std::min<uint32_t>(1024, ((L.nx + 63) / 64) * 64);
-    for (uint32_t bx = row; bx >= 64; bx /= 2) {
-        if (bx % 64) break;
+    std::vector<std::pair<uint32_t, uint32_t>> out{{L.bx, L.by}};
+    const uint32_t row = std::min<uint32_t>(1024, ((L.nx + 63) / 64) * 64);
+    std::vector<uint32_t> widths;
+    for (uint32_t k = 1; k <= row / 64; ++k)
+        if (row % k == 0 && (row / k) % 64 == 0) widths.push_back(row / k);
+    for (uint32_t bx : {64u, 128u, 256u})
+        if (bx < row && std::find(widths.begin(), widths.end(), bx) == widths.end()) widths.push_back(bx);
+    for (uint32_t bx : widths) {
         for (uint32_t t = 64; t <= 1024; t *= 2) {
             if (t < bx) continue;
             const uint32_t by = std::min<uint32_t>(t / bx, L.ny);
             if (std::find(out.begin(), out.end(), std::make_pair(bx, by)) == out.end()) out.push_back({bx, by});
         }
-        if (bx == 64) break;
     }
     return out;
 }
