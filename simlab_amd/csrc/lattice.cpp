@@ -185,8 +185,9 @@ const char *validate(const simlab_lattice_setup *s, std::string &msg) {
 // kernel per step (special and bulk blocks together: no cross-stream wait,
 // which costs about 15 us a step), larger ones with the two kernels on two
 // streams (each at its own register count, so the bulk kernel keeps its
-// occupancy). Tiles: one SPECIAL_BLOCK-thread tile (16-bit: 128 x 2) in the
-// fused kernel; in the split launch 16-bit storages run best with 128 x 1
+// occupancy). Tiles: one SPECIAL_BLOCK-thread tile of 128 x 2 in the fused
+// kernel (FP32 on 256x128x128: 2 % faster than whole 256-cell rows); in the
+// split launch 16-bit storages run best with 128 x 1
 // (two waves per block, more blocks per CU), 32/64-bit ones with whole
 // rows. Measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
 // searches tiles and both launches per lattice.
@@ -200,9 +201,8 @@ void choose_shape(DevLattice &L, uint32_t target_threads, uint32_t by_override, 
     const bool half = L.storage == SIMLAB_STORAGE_FP16 || L.storage == SIMLAB_STORAGE_BF16;
     if (!std::getenv("SIMLAB_CFD_BLOCK")) {
         if (fused) {
-            bx = std::min<uint32_t>(bx, half ? 128 : SPECIAL_BLOCK);
+            bx = std::min<uint32_t>(bx, 128);   // 64 or 128: divides SPECIAL_BLOCK
             target_threads = SPECIAL_BLOCK;
-            if (bx == 192) bx = 128;   // the tile must divide SPECIAL_BLOCK
         } else if (half && bx > 128) {
             bx = 128;
             target_threads = 128;
