@@ -11,10 +11,11 @@ per step, as the reference does (crates/cfd/examples/bench.rs:25).
 
 Reported (one JSON line):
   value            FP16-storage MLUPS (wall clock over the timed steps)
-  roofline         the bulk stream-collide kernel (k_bulk, the cells with no
-                   wall record and no face duty): algorithmic bytes per launch
-                   (2*19*s + 4 bytes per cell it updates, s = 2 for FP16) / its
-                   average duration from HIP events on the stream it runs on;
+  roofline         the stream-collide kernel (k_step, one launch per step):
+                   algorithmic bytes per launch (2*19*s + 4 bytes per fluid
+                   cell it updates, s = 2 for FP16) / its average duration
+                   from HIP events its own dispatch records
+                   (hipExtLaunchKernel, no packets between kernels);
                    storages.*.roofline_frac is the whole-step figure
                    (MLUPS over all cells x bytes per cell / 8 TB/s)
   fp32 / bf16      the same for FP32 and BF16 storage
@@ -50,8 +51,10 @@ def timed_run(sa, W, storage, steps, warmup):
     t0 = time.perf_counter()
     lat.run_still(steps)                      # returns after the stream is synchronized
     wall = time.perf_counter() - t0
+    # second pass: the stream-collide kernel's own start and end, recorded by
+    # its dispatch (hipExtLaunchKernel), and events around each chunk
     lat.set_timing(True)
-    lat.run_still(steps)                      # second pass: per-launch HIP events
+    lat.run_still(steps)
     step_ms, main_ms, launches = lat.timing()
     n = setup.cells()
     res = {
@@ -62,8 +65,11 @@ def timed_run(sa, W, storage, steps, warmup):
         "nrec": lat.nrec,
     }
     res["bulk_cells"] = lat.bulk_cells
+    # cells one launch updates: every fluid cell (bulk and wall-record cells)
+    res["kernel_cells"] = lat.bulk_cells + lat.nrec
     res["tile"] = list(lat.tile())
-    b = bytes_per_cell(storage) * lat.bulk_cells   # algorithmic bytes of one bulk launch
+    b = bytes_per_cell(storage) * res["kernel_cells"]   # algorithmic bytes of one launch
+    res["kernel_bytes"] = b
     res["kernel_gbs"] = b / (main_ms / launches * 1e-3) / 1e9
     res["kernel_roofline_frac"] = res["kernel_gbs"] / HBM_PEAK_GBS
     res["roofline_frac"] = res["mlups"] * 1e6 * bytes_per_cell(storage) / (HBM_PEAK_GBS * 1e9)
@@ -89,7 +95,7 @@ def large_runs(sa, W, steps, warmup):
         lat.set_timing(True)
         lat.run_still(steps)
         step_ms, main_ms, launches = lat.timing()
-        gbs = bytes_per_cell(st) * lat.bulk_cells / (main_ms / launches * 1e-3) / 1e9
+        gbs = bytes_per_cell(st) * (lat.bulk_cells + lat.nrec) / (main_ms / launches * 1e-3) / 1e9
         mlups = setup.cells() * steps / wall / 1e6
         out[f"sphere384_{name}"] = {
             "mlups": round(mlups, 1), "ms_per_step": round(wall / steps * 1e3, 4),
@@ -211,9 +217,25 @@ def drag_run(sa, W, storage, steps):
     return {"cd": cd, "cl": cl, "cd_last": float(h[-1, 0] * meta["cd_scale"]), "mass_end": m1}
 
 
-def cpu_baseline(sa, W, target_s=12.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sa, W, target_s=5.0, repeats=3):
+    """The CPU oracle on the bench workload: median MLUPS of `repeats` timed
+    samples of about target_s seconds each, on the host threads this process
+    may use (OMP_NUM_THREADS where it is set: the CPU share of one GPU on the
+    GPU box; else every core)."""
+    import statistics
     from oracle import pyoracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     setup, geom, meta = W.sphere_channel_large(sa.Storage.FP32)
     lat = O.OracleLattice(setup, geom, store=O.F32, threads=threads)
     lat.run_still(1)
@@ -221,23 +243,33 @@ def cpu_baseline(sa, W, target_s=12.0):
     lat.run_still(2)
     per = (time.perf_counter() - t0) / 2
     k = max(2, min(200, int(target_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    lat.run_still(k)
-    dt = time.perf_counter() - t0
     n = setup.cells()
-    return {"value": n * k / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"C++ oracle (restatement of the reference kernels), FP32 storage, {WORKLOAD}: {k} steps after 3 warm-up steps, {dt:.1f} s"}
+    vals = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        lat.run_still(k)
+        vals.append(n * k / (time.perf_counter() - t0) / 1e6)
+    return {"value": round(statistics.median(vals), 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "host_cpus": os.cpu_count(), "samples": [round(v, 2) for v in vals],
+            "sample": f"C++ oracle (restatement of the reference kernels), FP32 storage, {WORKLOAD}: median of "
+                      f"{repeats} runs of {k} steps after 3 warm-up steps, {threads} threads"}
 
 
-def traffic_from_profile():
+TRAFFIC_PROFILE = os.path.join("profiles", "stream_collide_sphere256_fp16_traffic.json")
+
+
+def traffic_from_profile(storage_name):
     """HBM bytes per stream-collide launch from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload (None if absent)."""
-    p = os.path.join(ROOT, "profiles", "stream_collide_sphere256_fp16_traffic.json")
+    FETCH_SIZE/WRITE_SIZE summary of this workload, for the storage it was
+    taken with (None if absent or for another storage)."""
     try:
-        with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    if d.get("storage", "fp16") != storage_name:
+        return None
+    return d.get("bytes_per_launch")
 
 
 def copy_calibration(reps):
@@ -360,7 +392,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(),
+            "traffic": traffic_from_profile(head_name),
+            "kernel": "k_step (one launch per step: stream-collide of every fluid cell, wall records, halfway "
+                      "bounce-back, FP64 force partials)",
+            "kernel_us": round(head["kernel_us"], 2),
+            "bytes_per_launch": head["kernel_bytes"],
+            "bytes": f"{bytes_per_cell(head_st)} B (19 loads + 19 stores of {head_st.pop_bytes()} B + a 4 B flag) x "
+                     f"{head['kernel_cells']} fluid cells the launch updates",
+            "traffic_source": f"{TRAFFIC_PROFILE} (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch)",
         },
         "cpu_baseline": cpu,
         "storages": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in results.items()},

@@ -167,7 +167,9 @@ typedef struct simlab_static_voxels simlab_static_voxels; /* opaque */
 /* Message of the last failed call on this thread ("" if none). */
 const char *simlab_cfd_last_error(void);
 
-/* `gpu_info` (gpu.rs:56-63): "name · HIP · gfx · GB max buffer". */
+/* `gpu_info` (gpu.rs:56-63): "name · HIP · N.N GB max buffer" (the device
+ * name, or AMD and its architecture where the driver gives none; the free
+ * device memory, the largest buffer one allocation can have). */
 int simlab_cfd_gpu_info(char *buf, size_t len);
 
 /* `Lattice::max_cells` (lattice.rs:223-227): most cells a lattice with this
@@ -236,11 +238,13 @@ int simlab_cfd_lattice_checkpoint_size(const simlab_lattice *lat, size_t *bytes)
 int simlab_cfd_lattice_checkpoint(simlab_lattice *lat, void *out, size_t bytes);
 int simlab_cfd_lattice_restore(simlab_lattice *lat, const void *in, size_t bytes);
 
-/* Device timing of the last `run` call: total milliseconds of its steps and
- * the summed duration and count of the stream-collide launches (the
- * one-kernel step, or the bulk kernel of the two-stream step), measured
- * with HIP events on the stream they run on (timing must have been
- * enabled). */
+/* Device timing of the last `run` call: total milliseconds of its steps
+ * (HIP events around each chunk of steps on the lattice's stream) and the
+ * summed duration and count of the stream-collide launches (the one-kernel
+ * step, or the bulk kernel of the two-stream step), from HIP events the
+ * kernel's own dispatch records (hipExtLaunchKernel: no packets between the
+ * kernels, so the figure is the kernel's own duration). Timing must have
+ * been enabled. */
 int simlab_cfd_lattice_set_timing(simlab_lattice *lat, int enable);
 int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms,
                               double *main_ms, uint64_t *main_launches);

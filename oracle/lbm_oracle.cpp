@@ -1077,6 +1077,20 @@ void oracle_set_fused(void *hp, int on) {
     DISPATCH(h, L->fused = on != 0);
 }
 
+// Only the face update that opens the next step (face_gather + face_write of
+// module p = steps_done % 2, shader.rs:630-687), without the step itself and
+// without advancing steps_done: the face cells' outgoing populations as the
+// main step would read them, for the face-rule tests.
+void oracle_face_update(void *hp) {
+    Handle *h = (Handle *)hp;
+    DISPATCH(h, {
+        const int p = (int)(L->steps_done % 2);
+        const uint32_t seed = (uint32_t)L->steps_done * 2654435761u;
+        for (uint32_t k = 0; k < (uint32_t)L->faces.size(); ++k) L->face_gather(k, p);
+        for (uint32_t k = 0; k < (uint32_t)L->faces.size(); ++k) L->face_write(k, p, seed);
+    });
+}
+
 int oracle_run(void *hp, const simlab_step_in *steps, uint64_t nsteps, double *ft) {
     Handle *h = (Handle *)hp;
     double tmp[6];

@@ -69,6 +69,7 @@ def lib():
         L.oracle_aa_maps.argtypes = [P, C.c_int, P, P]
         L.oracle_collide_f64.argtypes = [P, C.c_double, C.c_double, P, P]
         L.oracle_set_fused.argtypes = [V, C.c_int]
+        L.oracle_face_update.argtypes = [V]
         L.oracle_wall_tau_f64.restype = C.c_double
         L.oracle_wall_tau_f64.argtypes = [C.c_uint32, P, P, P, C.c_double, P, C.c_double]
         L.oracle_refill_stress_f64.argtypes = [P, P, C.c_int, P]
@@ -167,6 +168,11 @@ class OracleLattice:
         out = np.zeros(max(self.nface, 1), np.uint32)
         self._lib.oracle_get_faces(self._h, _ptr(out))
         return out[: self.nface]
+
+    def face_update(self):
+        """Only the face update that opens the next step (face_gather +
+        face_write), steps_done unchanged."""
+        self._lib.oracle_face_update(self._h)
 
     def set_fused(self, on: bool):
         self._lib.oracle_set_fused(self._h, int(bool(on)))

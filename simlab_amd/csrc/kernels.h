@@ -5,6 +5,7 @@
 #include "../../include/simlab_cfd.h"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 
 namespace simlab {
@@ -140,15 +141,40 @@ struct BatchLaunch {
 // special kernel of the batch on s_special, bulk kernel on s_bulk
 hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 
+// The launchers of one population storage (kernels.hip compiled once per
+// storage); the functions below dispatch on the lattice's storage.
+struct StorageLaunchers {
+    hipError_t (*init)(const DevLattice &, hipStream_t);
+    hipError_t (*faces)(const DevLattice &, int, uint32_t, hipStream_t);
+    hipError_t (*xfaces)(const DevLattice &, int, uint32_t, hipStream_t);
+    hipError_t (*stream_collide)(const DevLattice &, int, const StepArgs &, hipStream_t, hipStream_t, hipEvent_t,
+                                 hipEvent_t);
+    hipError_t (*step_fused)(const DevLattice &, int, const StepArgs &, hipStream_t, hipEvent_t, hipEvent_t);
+    hipError_t (*macros)(const DevLattice &, int, float4 *, hipStream_t);
+    hipError_t (*mass_partial)(const DevLattice &, int, double *, uint32_t, hipStream_t);
+    hipError_t (*samplers)(const DevLattice &, int, const StepArgs &, hipStream_t);
+    hipError_t (*pbb_step)(const DevLattice &, int, const StepArgs &, const float *, hipStream_t);
+    hipError_t (*adjoint_step)(const DevLattice &, const DevLattice &, int, const float *, double *, double, int,
+                               hipStream_t);
+    hipError_t (*batch_step)(const BatchLaunch &, int, const StepArgs &, hipStream_t, hipStream_t);
+    hipError_t (*voxelize)(const DevLattice &, int, const StepArgs &, hipStream_t);
+};
+extern const StorageLaunchers launchers_fp32, launchers_fp16, launchers_bf16, launchers_fp64;
+
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
 // standalone face update before a step of this parity (seed of that step)
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
 // the x-face cells from the sources' states the step left in xsrc
 hipError_t launch_xfaces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
-// k_special on s_special, k_bulk on s_bulk (independent within a step)
-hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
-// the same as one kernel on one stream (the tile's bx * by must divide SPECIAL_BLOCK)
-hipError_t launch_step_fused(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+// k_special on s_special, k_bulk on s_bulk (independent within a step).
+// With ev0/ev1 the bulk kernel's dispatch records its own start and end in
+// them (hipExtLaunchKernel: no extra packets between the kernels)
+hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk,
+                                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// the same as one kernel on one stream (the tile's bx * by must divide
+// SPECIAL_BLOCK); ev0/ev1 as above, for that kernel
+hipError_t launch_step_fused(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 // re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)

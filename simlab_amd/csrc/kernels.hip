@@ -28,6 +28,10 @@
 
 #include "kernels.h"
 
+#ifndef SIMLAB_KSTORE
+#error "SIMLAB_KSTORE: the storage compiled here (0-3), or -1 for the storage-independent part"
+#endif
+
 #include <cstdint>
 #include <type_traits>
 #include <utility>
@@ -893,6 +897,7 @@ __device__ __forceinline__ void reduce_ft(const DevLattice &L, const StepArgs &a
     }
 }
 
+#if SIMLAB_KSTORE < 0
 // force history of a chunk: block s sums the partials of step s
 __global__ void __launch_bounds__(256) k_reduce_hist(const double *partials, uint32_t nblocks, double *hist) {
     __shared__ double sh[16][6];
@@ -906,6 +911,7 @@ __global__ void __launch_bounds__(256) k_reduce_hist(const double *partials, uin
 #pragma unroll
         for (int j = 0; j < 6; ++j) hist[(size_t)s * HIST_STRIDE + j] = v[j];
 }
+#endif
 
 // ---------------------------------------------------------------- special cells
 // k_special processes, for a step of parity P, the wall-record cells, one
@@ -1317,6 +1323,7 @@ __device__ __forceinline__ float ray_tri(V3f o, V3f d, V3f v0, V3f e1, V3f e2) {
     return t > 0.0f ? t : 0.0f;
 }
 
+#if SIMLAB_KSTORE < 0
 // one wave per triangle; its lanes share the lattice cells of the
 // triangle's bounding box (grown by 1.415)
 __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
@@ -1391,6 +1398,7 @@ __global__ void __launch_bounds__(256) k_qclear_near(DevLattice L, StepArgs a) {
         o[3 * w + 2] = 0u;
     }
 }
+#endif
 
 // f16 round to nearest even of a q in [0, 1] (pack2x16float)
 __device__ __forceinline__ uint32_t half_rne_bits(float q) {
@@ -1959,6 +1967,7 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_mass_partial(
     __syncthreads();
     if (threadIdx.x == 0) partials[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
+#if SIMLAB_KSTORE < 0
 __global__ void __launch_bounds__(256) k_sum_final(const double *partials, uint32_t n, double *out) {
     __shared__ double sh[4];
     double s = 0.0;
@@ -1968,6 +1977,7 @@ __global__ void __launch_bounds__(256) k_sum_final(const double *partials, uint3
     __syncthreads();
     if (threadIdx.x == 0) *out = sh[0] + sh[1] + sh[2] + sh[3];
 }
+#endif
 
 // ---------------------------------------------------------------- probes
 // shader.rs:736-784 (run with the module of the other parity after a step)
@@ -2043,295 +2053,295 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_sample_slices
 // ---------------------------------------------------------------- dispatch helpers
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// a kernel launch, with its dispatch recording its own start and end in
+// ev0 / ev1 when they are given (hipExtLaunchKernel: the timestamps of the
+// dispatch itself, no packets between the kernels)
+template <class K, class... Args>
+hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
+    if (ev0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev0, ev1, 0, args...);
+    else kernel<<<grid, block, 0, s>>>(args...);
+    return hipGetLastError();
+}
+
 template <class C, int P, bool FORCE, bool SPONGE>
-hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk) {
+hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk, hipEvent_t ev0,
+                        hipEvent_t ev1) {
     k_special<C, P, FORCE, SPONGE><<<L.wall_blocks + L.hw_blocks, SPECIAL_BLOCK, 0, s_special>>>(L, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_bulk<C, P, FORCE, SPONGE><<<dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), 0, s_bulk>>>(L, a);
-    return hipGetLastError();
+    return launch_timed(k_bulk<C, P, FORCE, SPONGE>, dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), s_bulk, ev0, ev1,
+                        L, a);
 }
 
 template <class C, int P, bool FORCE, bool SPONGE>
-hipError_t fused_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+hipError_t fused_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     const uint64_t tiles = (uint64_t)L.xblocks * L.yblocks * L.nz;
     const uint32_t per = SPECIAL_BLOCK / (L.bx * L.by);
     const uint32_t nb = L.wall_blocks + L.hw_blocks + cdiv(tiles, per);
-    k_step<C, P, FORCE, SPONGE><<<nb, SPECIAL_BLOCK, 0, s>>>(L, a);
-    return hipGetLastError();
+    return launch_timed(k_step<C, P, FORCE, SPONGE>, dim3(nb), dim3(SPECIAL_BLOCK), s, ev0, ev1, L, a);
 }
 
 template <class C, int P>
-hipError_t fused_launch(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+hipError_t fused_launch(const DevLattice &L, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     const bool force = L.forcing != 0, sponge = L.sponge != 0;
-    if (!force && !sponge) return fused_launch4<C, P, false, false>(L, a, s);
-    if (force && !sponge) return fused_launch4<C, P, true, false>(L, a, s);
-    if (!force && sponge) return fused_launch4<C, P, false, true>(L, a, s);
-    return fused_launch4<C, P, true, true>(L, a, s);
+    if (!force && !sponge) return fused_launch4<C, P, false, false>(L, a, s, ev0, ev1);
+    if (force && !sponge) return fused_launch4<C, P, true, false>(L, a, s, ev0, ev1);
+    if (!force && sponge) return fused_launch4<C, P, false, true>(L, a, s, ev0, ev1);
+    return fused_launch4<C, P, true, true>(L, a, s, ev0, ev1);
 }
 
 template <class C>
-hipError_t fused_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    return p == 0 ? fused_launch<C, 0>(L, a, s) : fused_launch<C, 1>(L, a, s);
+hipError_t fused_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    return p == 0 ? fused_launch<C, 0>(L, a, s, ev0, ev1) : fused_launch<C, 1>(L, a, s, ev0, ev1);
 }
 
 template <class C, int P>
-hipError_t step_launch(const DevLattice &L, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+hipError_t step_launch(const DevLattice &L, const StepArgs &a, hipStream_t s1, hipStream_t s2, hipEvent_t ev0,
+                       hipEvent_t ev1) {
     const bool force = L.forcing != 0, sponge = L.sponge != 0;
-    if (!force && !sponge) return step_launch4<C, P, false, false>(L, a, s1, s2);
-    if (force && !sponge) return step_launch4<C, P, true, false>(L, a, s1, s2);
-    if (!force && sponge) return step_launch4<C, P, false, true>(L, a, s1, s2);
-    return step_launch4<C, P, true, true>(L, a, s1, s2);
+    if (!force && !sponge) return step_launch4<C, P, false, false>(L, a, s1, s2, ev0, ev1);
+    if (force && !sponge) return step_launch4<C, P, true, false>(L, a, s1, s2, ev0, ev1);
+    if (!force && sponge) return step_launch4<C, P, false, true>(L, a, s1, s2, ev0, ev1);
+    return step_launch4<C, P, true, true>(L, a, s1, s2, ev0, ev1);
 }
 
-template <class C> hipError_t step_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
-    return p == 0 ? step_launch<C, 0>(L, a, s1, s2) : step_launch<C, 1>(L, a, s1, s2);
+template <class C>
+hipError_t step_launch_p(const DevLattice &L, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2, hipEvent_t ev0,
+                         hipEvent_t ev1) {
+    return p == 0 ? step_launch<C, 0>(L, a, s1, s2, ev0, ev1) : step_launch<C, 1>(L, a, s1, s2, ev0, ev1);
 }
 
 }  // namespace
 
-hipError_t launch_init(const DevLattice &L, hipStream_t s) {
-    const uint32_t nb = cdiv(L.ncell, 256);
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: k_init<CF32><<<nb, 256, 0, s>>>(L); break;
-        case SIMLAB_STORAGE_FP16: k_init<CF16><<<nb, 256, 0, s>>>(L); break;
-        case SIMLAB_STORAGE_BF16: k_init<CBF16><<<nb, 256, 0, s>>>(L); break;
-        default: k_init<CF64><<<nb, 256, 0, s>>>(L); break;
-    }
+// ---------------------------------------------------------------- launchers
+// Every storage's kernels are compiled in a translation unit of their own
+// (SIMLAB_KSTORE = its simlab_storage value), which fills that storage's
+// table of launchers; the storage-independent kernels and the dispatch on
+// L.storage are compiled with SIMLAB_KSTORE = -1.
+#if SIMLAB_KSTORE >= 0
+
+namespace {
+#if SIMLAB_KSTORE == 0
+using KC = CF32;
+#elif SIMLAB_KSTORE == 1
+using KC = CF16;
+#elif SIMLAB_KSTORE == 2
+using KC = CBF16;
+#else
+using KC = CF64;
+#endif
+constexpr bool KC_WIDE = std::is_same_v<KC, CF32> || std::is_same_v<KC, CF64>;
+
+hipError_t t_init(const DevLattice &L, hipStream_t s) {
+    k_init<KC><<<cdiv(L.ncell, 256), 256, 0, s>>>(L);
     return hipGetLastError();
 }
 
-hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
+hipError_t t_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
     if (L.nface == 0) return hipSuccess;
     const uint32_t nb = cdiv(L.nface, 256);
-#define FACES(C)                                                    \
-    do {                                                            \
-        if (p == 0) k_faces<C, 0><<<nb, 256, 0, s>>>(L, seed);      \
-        else k_faces<C, 1><<<nb, 256, 0, s>>>(L, seed);             \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: FACES(CF32); break;
-        case SIMLAB_STORAGE_FP16: FACES(CF16); break;
-        case SIMLAB_STORAGE_BF16: FACES(CBF16); break;
-        default: FACES(CF64); break;
-    }
-#undef FACES
+    if (p == 0) k_faces<KC, 0><<<nb, 256, 0, s>>>(L, seed);
+    else k_faces<KC, 1><<<nb, 256, 0, s>>>(L, seed);
     return hipGetLastError();
 }
 
-hipError_t launch_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
+hipError_t t_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
     if (!L.xdefer) return hipSuccess;
-    const uint32_t nb = cdiv(2ull * L.nz * L.ny * 5, 256);
-#define XF(C)                                                       \
-    do {                                                            \
-        if (p == 0) k_xfaces<C, 0><<<nb, 256, 0, s>>>(L, seed);     \
-        else k_xfaces<C, 1><<<nb, 256, 0, s>>>(L, seed);            \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: XF(CF32); break;
-        case SIMLAB_STORAGE_FP16: XF(CF16); break;
-        case SIMLAB_STORAGE_BF16: XF(CBF16); break;
-        default: return hipErrorInvalidValue;
-    }
-#undef XF
-    return hipGetLastError();
-}
-
-hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk) {
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: return step_launch_p<CF32>(L, p, a, s_special, s_bulk);
-        case SIMLAB_STORAGE_FP16: return step_launch_p<CF16>(L, p, a, s_special, s_bulk);
-        case SIMLAB_STORAGE_BF16: return step_launch_p<CBF16>(L, p, a, s_special, s_bulk);
-        default: return step_launch_p<CF64>(L, p, a, s_special, s_bulk);
+    if constexpr (std::is_same_v<KC, CF64>) {
+        return hipErrorInvalidValue;
+    } else {
+        const uint32_t nb = cdiv(2ull * L.nz * L.ny * 5, 256);
+        if (p == 0) k_xfaces<KC, 0><<<nb, 256, 0, s>>>(L, seed);
+        else k_xfaces<KC, 1><<<nb, 256, 0, s>>>(L, seed);
+        return hipGetLastError();
     }
 }
 
-hipError_t launch_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+hipError_t t_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2, hipEvent_t ev0,
+                            hipEvent_t ev1) {
+    return step_launch_p<KC>(L, p, a, s1, s2, ev0, ev1);
+}
+
+hipError_t t_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     if (L.bx * L.by > SPECIAL_BLOCK || SPECIAL_BLOCK % (L.bx * L.by) != 0) return hipErrorInvalidConfiguration;
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: return fused_launch_p<CF32>(L, p, a, s);
-        case SIMLAB_STORAGE_FP16: return fused_launch_p<CF16>(L, p, a, s);
-        case SIMLAB_STORAGE_BF16: return fused_launch_p<CBF16>(L, p, a, s);
-        default: return fused_launch_p<CF64>(L, p, a, s);
-    }
+    return fused_launch_p<KC>(L, p, a, s, ev0, ev1);
 }
 
-hipError_t launch_macros(const DevLattice &L, int p, float4 *out, hipStream_t s) {
+hipError_t t_macros(const DevLattice &L, int p, float4 *out, hipStream_t s) {
     const uint32_t nb = cdiv(L.ncell, 256);
-#define MAC(C)                                                      \
-    do {                                                            \
-        if (p == 0) k_macros<C, 0><<<nb, 256, 0, s>>>(L, out);      \
-        else k_macros<C, 1><<<nb, 256, 0, s>>>(L, out);             \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: MAC(CF32); break;
-        case SIMLAB_STORAGE_FP16: MAC(CF16); break;
-        case SIMLAB_STORAGE_BF16: MAC(CBF16); break;
-        default: MAC(CF64); break;
-    }
-#undef MAC
+    if (p == 0) k_macros<KC, 0><<<nb, 256, 0, s>>>(L, out);
+    else k_macros<KC, 1><<<nb, 256, 0, s>>>(L, out);
     return hipGetLastError();
 }
 
-hipError_t launch_mass(const DevLattice &L, int p, double *partials, uint32_t nblocks, double *out, hipStream_t s) {
-#define MASS(C)                                                                \
-    do {                                                                       \
-        if (p == 0) k_mass_partial<C, 0><<<nblocks, 256, 0, s>>>(L, partials); \
-        else k_mass_partial<C, 1><<<nblocks, 256, 0, s>>>(L, partials);        \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: MASS(CF32); break;
-        case SIMLAB_STORAGE_FP16: MASS(CF16); break;
-        case SIMLAB_STORAGE_BF16: MASS(CBF16); break;
-        default: MASS(CF64); break;
+hipError_t t_mass_partial(const DevLattice &L, int p, double *partials, uint32_t nblocks, hipStream_t s) {
+    if (p == 0) k_mass_partial<KC, 0><<<nblocks, 256, 0, s>>>(L, partials);
+    else k_mass_partial<KC, 1><<<nblocks, 256, 0, s>>>(L, partials);
+    return hipGetLastError();
+}
+
+hipError_t t_samplers(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    // samplers use the module of the other parity (lattice.rs:733-737)
+    const int q = 1 - p;
+    if (L.nvert) {
+        const uint32_t nbv = cdiv((uint64_t)L.nvert * 8, 256);
+        if (q == 0) k_sample_verts<KC, 0><<<nbv, 256, 0, s>>>(L, a);
+        else k_sample_verts<KC, 1><<<nbv, 256, 0, s>>>(L, a);
     }
-#undef MASS
+    if (L.npix) {
+        if (q == 0) k_sample_slices<KC, 0><<<cdiv(L.npix, 256), 256, 0, s>>>(L);
+        else k_sample_slices<KC, 1><<<cdiv(L.npix, 256), 256, 0, s>>>(L);
+    }
+    return hipGetLastError();
+}
+
+hipError_t t_pbb_step(const DevLattice &L, int p, const StepArgs &a, const float *phi, hipStream_t s) {
+    if constexpr (!KC_WIDE) {
+        return hipErrorInvalidValue;
+    } else {
+        const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
+        if (p == 0) k_pbb_step<KC, 0><<<grid, block, 0, s>>>(L, a, phi);
+        else k_pbb_step<KC, 1><<<grid, block, 0, s>>>(L, a, phi);
+        return hipGetLastError();
+    }
+}
+
+hipError_t t_adjoint_step(const DevLattice &L, const DevLattice &A, int p, const float *phi, double *grad, double wj,
+                          int axis, hipStream_t s) {
+    if constexpr (!KC_WIDE) {
+        return hipErrorInvalidValue;
+    } else {
+        const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
+        const uint32_t nb = cdiv(L.ncell, 256);
+        if (p == 0) {
+            k_adj_faces<KC, 0><<<nb, 256, 0, s>>>(L, A, phi);
+            k_adj_step<KC, 0><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);
+        } else {
+            k_adj_faces<KC, 1><<<nb, 256, 0, s>>>(L, A, phi);
+            k_adj_step<KC, 1><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);
+        }
+        return hipGetLastError();
+    }
+}
+
+template <int P, bool FORCE, bool SPONGE>
+hipError_t batch_launch4(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    k_special_batch<KC, P, FORCE, SPONGE><<<b.nsp, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_bulk_batch<KC, P, FORCE, SPONGE><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
+    return hipGetLastError();
+}
+template <int P> hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    if (!b.force && !b.sponge) return batch_launch4<P, false, false>(b, a, s1, s2);
+    if (b.force && !b.sponge) return batch_launch4<P, true, false>(b, a, s1, s2);
+    if (!b.force && b.sponge) return batch_launch4<P, false, true>(b, a, s1, s2);
+    return batch_launch4<P, true, true>(b, a, s1, s2);
+}
+hipError_t t_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    return p == 0 ? batch_launch2<0>(b, a, s1, s2) : batch_launch2<1>(b, a, s1, s2);
+}
+
+hipError_t t_voxelize(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    const uint32_t nb = cdiv(L.nrec, 256);
+    if (p == 0) k_voxelize<KC, 0><<<nb, 256, 0, s>>>(L, a);
+    else k_voxelize<KC, 1><<<nb, 256, 0, s>>>(L, a);
+    return hipGetLastError();
+}
+}  // namespace
+
+#if SIMLAB_KSTORE == 0
+#define KTABLE launchers_fp32
+#elif SIMLAB_KSTORE == 1
+#define KTABLE launchers_fp16
+#elif SIMLAB_KSTORE == 2
+#define KTABLE launchers_bf16
+#else
+#define KTABLE launchers_fp64
+#endif
+#ifndef __HIP_DEVICE_COMPILE__   // a host table of host functions
+extern const StorageLaunchers KTABLE;
+const StorageLaunchers KTABLE = {t_init,       t_faces,          t_xfaces,   t_stream_collide, t_step_fused,
+                                 t_macros,     t_mass_partial,   t_samplers, t_pbb_step,       t_adjoint_step,
+                                 t_batch_step, t_voxelize};
+#endif
+#undef KTABLE
+
+#else   // SIMLAB_KSTORE < 0: storage-independent kernels and the dispatch
+
+namespace {
+const StorageLaunchers &kl(int32_t storage) {
+    switch (storage) {
+        case SIMLAB_STORAGE_FP32: return launchers_fp32;
+        case SIMLAB_STORAGE_FP16: return launchers_fp16;
+        case SIMLAB_STORAGE_BF16: return launchers_bf16;
+        default: return launchers_fp64;
+    }
+}
+}  // namespace
+
+hipError_t launch_init(const DevLattice &L, hipStream_t s) { return kl(L.storage).init(L, s); }
+hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) { return kl(L.storage).faces(L, p, seed, s); }
+hipError_t launch_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
+    return kl(L.storage).xfaces(L, p, seed, s);
+}
+hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk,
+                                 hipEvent_t ev0, hipEvent_t ev1) {
+    return kl(L.storage).stream_collide(L, p, a, s_special, s_bulk, ev0, ev1);
+}
+hipError_t launch_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    return kl(L.storage).step_fused(L, p, a, s, ev0, ev1);
+}
+hipError_t launch_macros(const DevLattice &L, int p, float4 *out, hipStream_t s) { return kl(L.storage).macros(L, p, out, s); }
+hipError_t launch_mass(const DevLattice &L, int p, double *partials, uint32_t nblocks, double *out, hipStream_t s) {
+    hipError_t e = kl(L.storage).mass_partial(L, p, partials, nblocks, s);
     if (e != hipSuccess) return e;
     k_sum_final<<<1, 256, 0, s>>>(partials, nblocks, out);
     return hipGetLastError();
 }
-
 hipError_t launch_reduce_hist(const DevLattice &L, uint32_t nsteps, hipStream_t s) {
     k_reduce_hist<<<nsteps, 256, 0, s>>>(L.partials, L.wall_blocks + L.hw_blocks, L.hist);
     return hipGetLastError();
 }
-
 hipError_t launch_reduce_partials(const double *partials, uint32_t nparts, uint32_t nsteps, double *hist, hipStream_t s) {
     k_reduce_hist<<<nsteps, 256, 0, s>>>(partials, nparts, hist);
     return hipGetLastError();
 }
-
 hipError_t launch_samplers(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    // samplers use the module of the other parity (lattice.rs:733-737)
-    const int q = 1 - p;
-#define SAMP(C)                                                                                   \
-    do {                                                                                          \
-        if (L.nvert) {                                                                            \
-            const uint32_t nbv = cdiv((uint64_t)L.nvert * 8, 256);                                \
-            if (q == 0) k_sample_verts<C, 0><<<nbv, 256, 0, s>>>(L, a);                           \
-            else k_sample_verts<C, 1><<<nbv, 256, 0, s>>>(L, a);                                  \
-        }                                                                                         \
-        if (L.npix) {                                                                             \
-            if (q == 0) k_sample_slices<C, 0><<<cdiv(L.npix, 256), 256, 0, s>>>(L);               \
-            else k_sample_slices<C, 1><<<cdiv(L.npix, 256), 256, 0, s>>>(L);                      \
-        }                                                                                         \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: SAMP(CF32); break;
-        case SIMLAB_STORAGE_FP16: SAMP(CF16); break;
-        case SIMLAB_STORAGE_BF16: SAMP(CBF16); break;
-        default: SAMP(CF64); break;
-    }
-#undef SAMP
-    return hipGetLastError();
+    return kl(L.storage).samplers(L, p, a, s);
 }
-
 hipError_t launch_pbb_step(const DevLattice &L, int p, const StepArgs &a, const float *phi, hipStream_t s) {
-    const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32:
-            if (p == 0) k_pbb_step<CF32, 0><<<grid, block, 0, s>>>(L, a, phi);
-            else k_pbb_step<CF32, 1><<<grid, block, 0, s>>>(L, a, phi);
-            break;
-        case SIMLAB_STORAGE_FP64:
-            if (p == 0) k_pbb_step<CF64, 0><<<grid, block, 0, s>>>(L, a, phi);
-            else k_pbb_step<CF64, 1><<<grid, block, 0, s>>>(L, a, phi);
-            break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return kl(L.storage).pbb_step(L, p, a, phi, s);
 }
-
 hipError_t launch_adjoint_step(const DevLattice &L, const DevLattice &A, int p, const float *phi, double *grad, double wj,
                                int axis, hipStream_t s) {
-    const dim3 grid(L.xblocks * L.yblocks, L.nz), block(L.bx, L.by);
-    const uint32_t nb = cdiv(L.ncell, 256);
-#define ADJ(C)                                                                                  \
-    do {                                                                                        \
-        if (p == 0) {                                                                           \
-            k_adj_faces<C, 0><<<nb, 256, 0, s>>>(L, A, phi);                                    \
-            k_adj_step<C, 0><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);                 \
-        } else {                                                                                \
-            k_adj_faces<C, 1><<<nb, 256, 0, s>>>(L, A, phi);                                    \
-            k_adj_step<C, 1><<<grid, block, 0, s>>>(L, A, phi, grad, wj, axis);                 \
-        }                                                                                       \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: ADJ(CF32); break;
-        case SIMLAB_STORAGE_FP64: ADJ(CF64); break;
-        default: return hipErrorInvalidValue;
-    }
-#undef ADJ
-    return hipGetLastError();
+    return kl(L.storage).adjoint_step(L, A, p, phi, grad, wj, axis, s);
 }
-
-namespace {
-template <class C, int P, bool FORCE, bool SPONGE>
-hipError_t batch_launch4(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
-    k_special_batch<C, P, FORCE, SPONGE><<<b.nsp, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_bulk_batch<C, P, FORCE, SPONGE><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
-    return hipGetLastError();
-}
-template <class C, int P>
-hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
-    if (!b.force && !b.sponge) return batch_launch4<C, P, false, false>(b, a, s1, s2);
-    if (b.force && !b.sponge) return batch_launch4<C, P, true, false>(b, a, s1, s2);
-    if (!b.force && b.sponge) return batch_launch4<C, P, false, true>(b, a, s1, s2);
-    return batch_launch4<C, P, true, true>(b, a, s1, s2);
-}
-}  // namespace
-
 hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
-#define BATCH(C) return p == 0 ? batch_launch2<C, 0>(b, a, s1, s2) : batch_launch2<C, 1>(b, a, s1, s2)
-    switch (b.storage) {
-        case SIMLAB_STORAGE_FP32: BATCH(CF32);
-        case SIMLAB_STORAGE_FP16: BATCH(CF16);
-        case SIMLAB_STORAGE_BF16: BATCH(CBF16);
-        default: BATCH(CF64);
-    }
-#undef BATCH
+    return kl(b.storage).batch_step(b, p, a, s1, s2);
 }
-
 hipError_t launch_scatter(const DevLattice &L, const StepArgs &a, hipStream_t s) {
     if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 4), 256, 0, s>>>(L, a);
     return hipGetLastError();
 }
-
 hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_t s) {
     k_qclear_near<<<cdiv(L.nrec, 256), 256, 0, s>>>(L, a);
     return hipGetLastError();
 }
-
 hipError_t launch_voxelize(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    const uint32_t nb = cdiv(L.nrec, 256);
-#define VOX(C)                                                          \
-    do {                                                                \
-        if (p == 0) k_voxelize<C, 0><<<nb, 256, 0, s>>>(L, a);          \
-        else k_voxelize<C, 1><<<nb, 256, 0, s>>>(L, a);                 \
-    } while (0)
-    switch (L.storage) {
-        case SIMLAB_STORAGE_FP32: VOX(CF32); break;
-        case SIMLAB_STORAGE_FP16: VOX(CF16); break;
-        case SIMLAB_STORAGE_BF16: VOX(CBF16); break;
-        default: VOX(CF64); break;
-    }
-#undef VOX
-    return hipGetLastError();
+    return kl(L.storage).voxelize(L, p, a, s);
 }
-
 hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
     hipError_t e = launch_scatter(L, a, s);
     if (e == hipSuccess) e = launch_voxelize(L, p, a, s);
     if (e == hipSuccess) e = launch_qclear_near(L, a, s);
     return e;
 }
-
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s) {
     k_qclear<<<cdiv((uint64_t)L.nrec * 9, 256), 256, 0, s>>>(L);
     return hipGetLastError();
 }
+
+#endif  // SIMLAB_KSTORE
 
 }  // namespace simlab

@@ -185,8 +185,9 @@ const char *validate(const simlab_lattice_setup *s, std::string &msg) {
 // kernel per step (special and bulk blocks together: no cross-stream wait,
 // which costs about 15 us a step), larger ones with the two kernels on two
 // streams (each at its own register count, so the bulk kernel keeps its
-// occupancy). Tiles: one SPECIAL_BLOCK-thread tile of 128 x 2 in the fused
-// kernel (FP32 on 256x128x128: 2 % faster than whole 256-cell rows); in the
+// occupancy). Tiles: two rows of bx cells in the fused kernel, bx = 128, or
+// 64 where that leaves fewer idle lanes in a row (FP32 on 256x128x128:
+// 128 x 2 is 2 % faster than whole 256-cell rows); in the
 // split launch 16-bit storages run best with 128 x 1
 // (two waves per block, more blocks per CU), 32/64-bit ones with whole
 // rows. Measured on 256x128x128 and 384^3; `simlab_cfd_lattice_autotune`
@@ -769,11 +770,12 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                 hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
             }
             const uint32_t tt = lat->L.bx * lat->L.by;
+            // with timing on, the stream-collide kernel's dispatch records its
+            // own start and end (no event packets between the kernels)
+            hipEvent_t e0 = lat->timing ? lat->ev[2 * k] : nullptr, e1 = lat->timing ? lat->ev[2 * k + 1] : nullptr;
             if (lat->fused_launch && tt <= SPECIAL_BLOCK && SPECIAL_BLOCK % tt == 0) {
                 // special and bulk blocks in one launch: no cross-stream wait
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
-                hip_check(launch_step_fused(lat->L, par, a, st), "stream-collide kernel");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+                hip_check(launch_step_fused(lat->L, par, a, st, e0, e1), "stream-collide kernel");
             } else if (lat->concurrent) {
                 // k_special on the main stream, k_bulk on the side stream
                 // after the fork: the special kernel is dispatched first, so
@@ -781,15 +783,11 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                 // and it ends within the bulk kernel instead of after it
                 hip_check(hipEventRecord(lat->fork, st), "event");
                 hip_check(hipStreamWaitEvent(lat->stream2, lat->fork, 0), "event wait");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], lat->stream2), "event");
-                hip_check(launch_stream_collide(lat->L, par, a, st, lat->stream2), "stream-collide kernels");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], lat->stream2), "event");
+                hip_check(launch_stream_collide(lat->L, par, a, st, lat->stream2, e0, e1), "stream-collide kernels");
                 hip_check(hipEventRecord(lat->join, lat->stream2), "event");
                 hip_check(hipStreamWaitEvent(st, lat->join, 0), "event wait");
             } else {
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k], st), "event");
-                hip_check(launch_stream_collide(lat->L, par, a, st, st), "stream-collide kernels");
-                if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * k + 1], st), "event");
+                hip_check(launch_stream_collide(lat->L, par, a, st, st, e0, e1), "stream-collide kernels");
             }
             if (lat->L.xdefer) hip_check(launch_xfaces(lat->L, par, a.seed + 2654435761u, st), "x-face kernel");
             if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
@@ -835,8 +833,20 @@ int simlab_cfd_gpu_info(char *buf, size_t len) {
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hipDeviceProp_t p;
         hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
-        std::snprintf(buf, len, "%s \xc2\xb7 HIP \xc2\xb7 %s \xc2\xb7 %.1f GB max buffer", p.name, p.gcnArchName,
-                      (double)p.totalGlobalMem / 1e9);
+        // gpu.rs:56-63: "name · backend · GB max buffer". The device name can
+        // be blank in the properties; then the architecture names the device
+        std::string name(p.name, strnlen(p.name, sizeof p.name));
+        while (!name.empty() && name.back() == ' ') name.pop_back();
+        if (name.empty()) {
+            std::string arch(p.gcnArchName, strnlen(p.gcnArchName, sizeof p.gcnArchName));
+            arch = arch.substr(0, arch.find(':'));
+            name = "AMD " + (arch.empty() ? std::string("GPU") : arch);
+        }
+        // the largest buffer one allocation can have now: the free device
+        // memory (a HIP allocation is not capped below it)
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        std::snprintf(buf, len, "%s \xc2\xb7 HIP \xc2\xb7 %.1f GB max buffer", name.c_str(), (double)free_b / 1e9);
         return SIMLAB_OK;
     });
 }
@@ -1094,12 +1104,15 @@ int simlab_cfd_lattice_timing(const simlab_lattice *lat, double *step_ms, double
 }  // extern "C"
 namespace {
 
+// a tuned choice holds for lattices of the same size and storage that run
+// the same kernel variant (walls, moving part, body force, sponge)
 struct TuneKey {
     uint32_t nx, ny, nz;
     int32_t storage;
-    bool walls;
+    bool walls, moving, forcing, sponge;
     bool operator<(const TuneKey &o) const {
-        return std::tie(nx, ny, nz, storage, walls) < std::tie(o.nx, o.ny, o.nz, o.storage, o.walls);
+        return std::tie(nx, ny, nz, storage, walls, moving, forcing, sponge) <
+               std::tie(o.nx, o.ny, o.nz, o.storage, o.walls, o.moving, o.forcing, o.sponge);
     }
 };
 std::mutex g_tune_mu;
@@ -1130,6 +1143,58 @@ std::vector<std::pair<uint32_t, uint32_t>> tile_candidates(const DevLattice &L) 
     return out;
 }
 
+// Keeps the evolving state of a lattice (every buffer a checkpoint holds,
+// plus a spinning part's hit scratch) in device copies while the autotuner
+// steps it, and restores it, steps_done, timing, tile and launch mode when it
+// goes out of scope; keep_tile() makes a new tile and launch mode stay.
+struct TuneGuard {
+    simlab_lattice *lat;
+    std::vector<DevBuf *> bufs;
+    std::vector<std::unique_ptr<DevBuf>> saved;
+    uint64_t steps0;
+    bool timing0, fused0;
+    uint32_t bx0, by0;
+    explicit TuneGuard(simlab_lattice *l)
+        : lat(l), steps0(l->steps_done), timing0(l->timing), fused0(l->fused_launch), bx0(l->L.bx), by0(l->L.by) {
+        bufs = {&l->pops, &l->flags, &l->rmask, &l->rq, &l->rfpost, &l->rmac, &l->rft, &l->vacc, &l->sacc, &l->qscr};
+        uint64_t mem = 0;
+        for (DevBuf *b : bufs) {
+            saved.emplace_back(new DevBuf);
+            if (b->p) {
+                saved.back()->alloc(b->bytes, mem);
+                hip_check(hipMemcpyAsync(saved.back()->p, b->p, b->bytes, hipMemcpyDeviceToDevice, l->stream), "copy");
+            }
+        }
+    }
+    void put_back_state() {
+        if (lat->scat_ready) {
+            // hits of a next step the trial launched on the third stream
+            hip_check(hipStreamSynchronize(lat->stream3), "GPU wait failed");
+            lat->scat_ready = false;
+        }
+        for (size_t k = 0; k < bufs.size(); ++k)
+            if (bufs[k]->p)
+                hip_check(hipMemcpyAsync(bufs[k]->p, saved[k]->p, bufs[k]->bytes, hipMemcpyDeviceToDevice, lat->stream),
+                          "copy");
+        lat->steps_done = steps0;
+    }
+    void keep_tile(uint32_t bx, uint32_t by, bool fused) {
+        bx0 = bx;
+        by0 = by;
+        fused0 = fused;
+    }
+    ~TuneGuard() {
+        try {
+            put_back_state();
+            hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
+        } catch (...) {
+        }
+        lat->timing = timing0;
+        set_shape(lat->L, bx0, by0);
+        lat->fused_launch = fused0;
+    }
+};
+
 }  // namespace
 extern "C" {
 
@@ -1140,7 +1205,8 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
         try {
             hip_check(hipSetDevice(lat->device), "hipSetDevice");
             DevLattice &L = lat->L;
-            const TuneKey key{L.nx, L.ny, L.nz, L.storage, lat->nrec != 0};
+            const TuneKey key{L.nx, L.ny, L.nz, L.storage, lat->nrec != 0 || lat->nhalfway != 0, lat->moving,
+                              L.forcing != 0, L.sponge != 0};
             if (use_cache) {
                 std::lock_guard<std::mutex> g(g_tune_mu);
                 auto it = g_tune.find(key);
@@ -1152,24 +1218,12 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
                     return SIMLAB_OK;
                 }
             }
-            // the trial steps advance the state: keep it and put it back
+            // the trial steps advance the state: keep every buffer a
+            // checkpoint holds and put it back after each trial; on any exit
+            // (a failed trial throws) the guard restores the state, steps_done,
+            // timing, tile and launch mode
             hipStream_t st = lat->stream;
-            const size_t pb = pop_elems(lat->ncell) * elem_bytes(lat->storage);
-            DevBuf save_pops, save_fpost, save_mac, save_ft;
-            uint64_t mem = 0;
-            save_pops.alloc(pb, mem);
-            save_fpost.alloc(lat->rfpost.bytes, mem);
-            save_mac.alloc(lat->rmac.bytes, mem);
-            save_ft.alloc(lat->rft.bytes, mem);
-            auto copy = [&](void *d, const void *s2, size_t b) {
-                hip_check(hipMemcpyAsync(d, s2, b, hipMemcpyDeviceToDevice, st), "copy");
-            };
-            copy(save_pops.p, lat->pops.p, pb);
-            copy(save_fpost.p, lat->rfpost.p, lat->rfpost.bytes);
-            copy(save_mac.p, lat->rmac.p, lat->rmac.bytes);
-            copy(save_ft.p, lat->rft.p, lat->rft.bytes);
-            const uint64_t steps0 = lat->steps_done;
-            const bool timing0 = lat->timing;
+            TuneGuard guard(lat);
             lat->timing = true;
             double best = 1e30;
             Tuned best_tile{L.bx, L.by, lat->fused_launch};
@@ -1179,29 +1233,25 @@ int simlab_cfd_lattice_autotune(simlab_lattice *lat, uint32_t steps, int use_cac
                     if (fz && (tt > SPECIAL_BLOCK || SPECIAL_BLOCK % tt != 0)) continue;
                     set_shape(L, c.first, c.second);
                     lat->fused_launch = fz != 0;
-                    if (run_steps(lat, nullptr, 2, nullptr, true) != SIMLAB_OK) continue;   // warm-up
-                    if (run_steps(lat, nullptr, steps, nullptr, true) != SIMLAB_OK) continue;
-                    const double us = lat->last_step_ms * 1e3 / steps;
-                    if (us < best) {
-                        best = us;
-                        best_tile = Tuned{c.first, c.second, fz != 0};
+                    const int r1 = run_steps(lat, nullptr, 2, nullptr, true);   // warm-up
+                    const int r2 = r1 == SIMLAB_OK ? run_steps(lat, nullptr, steps, nullptr, true) : r1;
+                    if (r2 == SIMLAB_OK) {
+                        const double us = lat->last_step_ms * 1e3 / steps;
+                        if (us < best) {
+                            best = us;
+                            best_tile = Tuned{c.first, c.second, fz != 0};
+                        }
                     }
-                    copy(lat->pops.p, save_pops.p, pb);
-                    copy(lat->rfpost.p, save_fpost.p, lat->rfpost.bytes);
-                    copy(lat->rmac.p, save_mac.p, lat->rmac.bytes);
-                    copy(lat->rft.p, save_ft.p, lat->rft.bytes);
-                    lat->steps_done = steps0;
+                    guard.put_back_state();
                 }
             }
             hip_check(hipStreamSynchronize(st), "GPU wait failed");
-            lat->timing = timing0;
-            set_shape(L, best_tile.bx, best_tile.by);
-            lat->fused_launch = best_tile.fused;
+            guard.keep_tile(best_tile.bx, best_tile.by, best_tile.fused);
             {
                 std::lock_guard<std::mutex> g(g_tune_mu);
                 g_tune[key] = best_tile;
             }
-            if (tile_out) { tile_out[0] = L.bx; tile_out[1] = L.by; }
+            if (tile_out) { tile_out[0] = best_tile.bx; tile_out[1] = best_tile.by; }
             if (step_us_out) *step_us_out = best;
             return SIMLAB_OK;
         } catch (const HipError &e) {

@@ -1,0 +1,243 @@
+"""The benchmark configurations at their full sizes, on the GPU through the
+C ABI:
+
+- sphere channel 128x64x64 (r 8 at x 32, Re 100, FP32, 2000 steps) against
+  the FP32 oracle run in the same test, one-kernel and two-stream launches:
+  velocity field (max and L2, relative to the inlet speed) and drag;
+- sphere channel 256x128x128 (r 16 at x 64, Re 1000, Cs 0.1, 5000 steps) in
+  FP32, FP16 and BF16 storage against an FP64-storage run: time-averaged
+  drag (the 0.5 % bar), the drag and lift time series, the velocity field at
+  the end; plus 20 steps of FP32 and FP16 against the oracle at full size;
+- the drone-arm tunnel 512x256x256 (Cs 0.16, u 0.06, 10000 steps) in FP16 and
+  FP32 against FP64: drag and the FP64 mass sum; the arm at 128^3, 192^3,
+  256^3 and 384^3 stepped as one batch against each lattice stepped alone,
+  and the 128^3 member against the oracle.
+
+Each test prints its figures on one line starting with "CONFIG"."""
+import json
+
+import numpy as np
+import pytest
+
+import simlab_amd as sa
+from simlab_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ORA = {sa.Storage.FP32: O.F32, sa.Storage.FP16: O.F16, sa.Storage.BF16: O.BF16, sa.Storage.FP64: O.F64}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def report(name, **kw):
+    say("CONFIG " + json.dumps({"case": name, **{k: (round(v, 8) if isinstance(v, float) else v) for k, v in kw.items()}}))
+
+
+def field_error(mg, mr, u_ref):
+    """Velocity field error of mg against mr over the cells both call fluid:
+    max |du| / u_ref and the L2 norm of du relative to that of u."""
+    ok = ~np.isnan(mr[:, 0]) & ~np.isnan(mg[:, 0])
+    du = mg[ok, 1:].astype(np.float64) - mr[ok, 1:].astype(np.float64)
+    return float(np.abs(du).max() / u_ref), float(np.linalg.norm(du) / np.linalg.norm(mr[ok, 1:].astype(np.float64)))
+
+
+# ---------------------------------------------------------------- C1
+C1_STEPS = 2000
+
+
+@pytest.fixture(scope="module")
+def c1_oracle():
+    setup, geom, meta = W.sphere_channel_small(sa.Storage.FP32, debug_dump=True)
+    say(f"oracle: sphere channel 128x64x64 FP32, {C1_STEPS} steps")
+    ora = O.OracleLattice(setup, geom, store=O.F32)
+    h = ora.run_still(C1_STEPS)
+    return h, ora.macros(), meta
+
+
+@pytest.mark.parametrize("launch", ["fused", "split"])
+def test_c1_sphere_channel_matches_oracle(c1_oracle, launch, monkeypatch):
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", launch)
+    ho, mo, meta = c1_oracle
+    setup, geom, _ = W.sphere_channel_small(sa.Storage.FP32, debug_dump=True)
+    say(f"case c1 {launch} launch")
+    lat = sa.Lattice(setup, geom)
+    hg = lat.run_still(C1_STEPS)
+    mg = lat.macros()
+    assert np.array_equal(np.isnan(mg[:, 0]), np.isnan(mo[:, 0]))
+    umax, l2 = field_error(mg, mo, meta["u"])
+    w = slice(C1_STEPS // 2, None)
+    cd_g, cd_o = hg[w, 0].mean() * meta["cd_scale"], ho[w, 0].mean() * meta["cd_scale"]
+    fmax = float(np.abs(hg - ho).max() / np.abs(ho).max())
+    report("c1_fp32_vs_oracle", launch=launch, steps=C1_STEPS, cd_gpu=cd_g, cd_oracle=cd_o, cd_rel=cd_g / cd_o - 1,
+           u_max_rel=umax, u_l2_rel=l2, force_max_rel=fmax)
+    assert abs(cd_g / cd_o - 1) < 1e-4
+    assert umax < 1e-3 and l2 < 1e-4
+    assert fmax < 1e-3
+
+
+# ---------------------------------------------------------------- C2
+C2_STEPS = 5000
+
+
+def c2_run(storage):
+    setup, geom, meta = W.sphere_channel_large(storage, debug_dump=True)
+    lat = sa.Lattice(setup, geom)
+    m0 = lat.mass()
+    h = lat.run_still(C2_STEPS)
+    return dict(h=h, mac=lat.macros(), mass0=m0, mass=lat.mass(), meta=meta)
+
+
+@pytest.fixture(scope="module")
+def c2_fp64():
+    say(f"reference run: sphere channel 256x128x128 FP64 storage, {C2_STEPS} steps")
+    return c2_run(sa.Storage.FP64)
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP32, sa.Storage.FP16, sa.Storage.BF16])
+def test_c2_storages_against_fp64(c2_fp64, storage):
+    say(f"case c2 {storage.name}, {C2_STEPS} steps")
+    r = c2_run(storage)
+    ref = c2_fp64
+    s = ref["meta"]["cd_scale"]
+    cd, cd64 = r["h"][:, 0] * s, ref["h"][:, 0] * s
+    w = slice(C2_STEPS // 2, None)
+    cd_mean, cd64_mean = cd[w].mean(), cd64[w].mean()
+    # lift and side force coefficients (z, y), and their time series
+    cl, cl64 = r["h"][:, 2] * s, ref["h"][:, 2] * s
+    cs_, cs64 = r["h"][:, 1] * s, ref["h"][:, 1] * s
+    dcd_t = np.abs(cd - cd64) / cd64_mean
+    dcl_t = np.maximum(np.abs(cl - cl64), np.abs(cs_ - cs64)) / cd64_mean
+    half = C2_STEPS // 2
+    umax, l2 = field_error(r["mac"], ref["mac"], ref["meta"]["u"])
+    report(f"c2_{storage.name.lower()}_vs_fp64", steps=C2_STEPS, cd=cd_mean, cd_fp64=cd64_mean,
+           cd_rel=cd_mean / cd64_mean - 1, cd_last_rel=cd[-1] / cd64[-1] - 1,
+           dcd_t_max_first_half=float(dcd_t[:half].max()), dcd_t_max=float(dcd_t.max()),
+           dcl_t_max_first_half=float(dcl_t[:half].max()), dcl_t_max=float(dcl_t.max()),
+           cl_mean=float(cl[w].mean()), cl_fp64_mean=float(cl64[w].mean()),
+           u_max_rel=umax, u_l2_rel=l2, mass_rel=r["mass"] / ref["mass"] - 1)
+    # the north star's bar: time-averaged drag within 0.5 % of the FP64 run
+    assert abs(cd_mean / cd64_mean - 1) < 0.005
+    # the series agree while the wake is still developing; once it sheds,
+    # rounding differences change the phase of the shedding, not its mean
+    assert dcd_t[:half].max() < {sa.Storage.FP32: 1e-4, sa.Storage.FP16: 2e-3, sa.Storage.BF16: 1e-2}[storage]
+    assert dcl_t[:half].max() < {sa.Storage.FP32: 1e-4, sa.Storage.FP16: 2e-3, sa.Storage.BF16: 1e-2}[storage]
+    assert l2 < {sa.Storage.FP32: 1e-2, sa.Storage.FP16: 5e-2, sa.Storage.BF16: 1e-1}[storage]
+    assert abs(r["mass"] / ref["mass"] - 1) < {sa.Storage.FP32: 1e-6, sa.Storage.FP16: 1e-4, sa.Storage.BF16: 1e-3}[storage]
+
+
+def live_post(pops, flags, dims, p):
+    fluid = np.nonzero((flags & 3) == 0)[0]
+    _, st = O.aa_maps(dims, p)
+    return pops[st[fluid].astype(np.int64)]
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP32, sa.Storage.FP16])
+def test_c2_full_size_against_oracle(storage):
+    # 20 steps at the full size, the record-free halfway bounce-back on (as
+    # the benchmark runs): FP32 populations within 1e-6 (relative to the
+    # inflow's deviations), FP16 bits identical but for the halfway wall
+    # cells' rounding (the library bounces back the stored 16-bit value,
+    # the reference an f32 copy) and its neighbours
+    setup, geom, meta = W.sphere_channel_large(storage, debug_dump=True)
+    say(f"case c2 vs oracle {storage.name}, 20 steps")
+    lat = sa.Lattice(setup, geom)
+    ora = O.OracleLattice(setup, geom, store=ORA[storage])
+    assert lat.halfway_cells > 0
+    hg = lat.run_still(20)
+    ho = ora.run_still(20)
+    flags = ora.flags()
+    pg = live_post(lat.get_pops(), flags, setup.dims, 1)
+    po = live_post(ora.get_pops(), flags, setup.dims, 1)
+    umax, l2 = field_error(lat.macros(), ora.macros(), meta["u"])
+    fmax = float(np.abs(hg - ho).max() / np.abs(ho).max())
+    if storage == sa.Storage.FP32:
+        d = float(np.abs(pg.astype(np.float64) - po.astype(np.float64)).max())
+        report("c2_fp32_vs_oracle_20", max_pop_diff=d, u_max_rel=umax, u_l2_rel=l2, force_max_rel=fmax)
+        assert d < 1e-6 and umax < 1e-4 and fmax < 1e-3
+    else:
+        diff = np.abs(pg.astype(np.int64) - po.astype(np.int64))
+        same = float(np.mean(diff == 0))
+        dv = float(np.abs(pg.view(np.float16).astype(np.float64) - po.view(np.float16).astype(np.float64)).max()) / 32768
+        report("c2_fp16_vs_oracle_20", identical=same, max_diff_ulp=int(diff.max()), max_decoded_diff=dv,
+               u_max_rel=umax, u_l2_rel=l2, force_max_rel=fmax)
+        # the deviations here are below 0.03; the stated bar is 4 units in
+        # the last place of FP16 at that size (2^-10 relative)
+        assert same > 0.95 and dv <= 4 * 0.03 * 2.0 ** -10
+        assert umax < 2e-3 and fmax < 2e-2
+
+
+# ---------------------------------------------------------------- C4
+C4_STEPS = 10000
+
+
+def c4_run(storage):
+    setup, geom, meta = W.drone_arm(storage=storage)
+    lat = sa.Lattice(setup, geom)
+    hs = []
+    for done in range(0, C4_STEPS, 2500):
+        hs.append(lat.run_still(2500))
+        say(f"  drone arm {storage.name}: {done + 2500} / {C4_STEPS} steps")
+    return dict(h=np.concatenate(hs), mass=lat.mass(), nrec=lat.nrec)
+
+
+@pytest.fixture(scope="module")
+def c4_fp64():
+    say(f"reference run: drone arm 512x256x256 FP64 storage, {C4_STEPS} steps")
+    return c4_run(sa.Storage.FP64)
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.FP32])
+def test_c4_drone_arm_against_fp64(c4_fp64, storage):
+    say(f"case c4 {storage.name}, {C4_STEPS} steps")
+    r = c4_run(storage)
+    w = slice(C4_STEPS // 2, None)
+    d, d64 = r["h"][w, 0].mean(), c4_fp64["h"][w, 0].mean()
+    side = np.hypot(r["h"][w, 1].mean() - c4_fp64["h"][w, 1].mean(), r["h"][w, 2].mean() - c4_fp64["h"][w, 2].mean())
+    report(f"c4_{storage.name.lower()}_vs_fp64", steps=C4_STEPS, drag=float(d), drag_fp64=float(d64),
+           drag_rel=float(d / d64 - 1), side_lift_rel=float(side / abs(d64)), mass_rel=r["mass"] / c4_fp64["mass"] - 1)
+    assert abs(d / d64 - 1) < 0.005
+    assert abs(r["mass"] / c4_fp64["mass"] - 1) < 1e-4
+
+
+C4_BATCH = (128, 192, 256, 384)
+
+
+def test_c4_batch_of_four_sizes(monkeypatch):
+    # the batch kernels write the x-face cells in the step itself; single
+    # lattices in 16-bit storage defer that to a small kernel (the same
+    # values up to the rounding of a multiply-add): the single runs here
+    # take the batch's path
+    monkeypatch.setenv("SIMLAB_CFD_XDEFER", "0")
+    built = [W.drone_arm(dims=(n, n, n), storage=sa.Storage.FP16, scale=n / 256.0) for n in C4_BATCH]
+    say("case c4 batch", C4_BATCH)
+    solo = [sa.Lattice(s, g) for s, g, _ in built]
+    members = [sa.Lattice(s, g) for s, g, _ in built]
+    for m in members:
+        m.autotune(steps=10, use_cache=True)
+    for a, m in zip(solo, members):
+        a.set_tile(*m.tile())
+    ft_solo = [np.concatenate([l.run_still(12), l.run_still(18)]) for l in solo]
+    b = sa.Batch(members)
+    ft = np.concatenate([b.run_still(12), b.run_still(18)], axis=1)
+    out = {}
+    for k, (a, m) in enumerate(zip(solo, members)):
+        assert m.steps_done == 30
+        pa, pm = a.get_pops(), m.get_pops()
+        same = float(np.mean(pa == pm))
+        dv = float(np.abs(pa.view(np.float16).astype(np.float64) - pm.view(np.float16).astype(np.float64)).max()) / 32768
+        out[str(C4_BATCH[k])] = dict(identical=same, max_diff=dv, tile=list(m.tile()))
+        assert same > 0.9 and dv < 0.02 * 2.0 ** -9
+        assert np.allclose(ft[k], ft_solo[k], rtol=1e-4, atol=1e-4 * np.abs(ft_solo[k]).max())
+    # the smallest member against the oracle (the reference's arithmetic)
+    s, g, _ = built[0]
+    ora = O.OracleLattice(s, g, store=O.F16)
+    ho = ora.run_still(30)
+    diff = np.abs(live_post(members[0].get_pops(), ora.flags(), s.dims, 1).astype(np.int64) -
+                  live_post(ora.get_pops(), ora.flags(), s.dims, 1).astype(np.int64))
+    fmax = float(np.abs(ft[0] - ho).max() / np.abs(ho).max())
+    report("c4_batch", members=out, oracle_identical=float(np.mean(diff == 0)), oracle_max_diff_ulp=int(diff.max()),
+           oracle_force_max_rel=fmax)
+    assert np.mean(diff == 0) > 0.95 and fmax < 1e-2

@@ -16,6 +16,10 @@ from simlab_amd.shapes import PropSpec
 
 pytestmark = pytest.mark.gpu
 
+# results the reference publishes for these studies (docs/VALIDATION.md)
+REF_CD_RE20 = 2.689
+REF_SPIN_TORQUE = 1.146e-3   # N m
+
 
 def _say(msg):
     print(msg, flush=True)
@@ -74,6 +78,9 @@ def test_sphere_re20():
          f"{s.simulated:.1f} passes, {s.mlups:.0f} MLUPS, {s.seconds:.1f} s")
     assert abs(cd / refc - 1.0) < 0.05
     assert math.hypot(s.lift_n, s.side_n) / abs(s.drag_n) < 0.05
+    # the reference's own result for this study (docs/VALIDATION.md:95):
+    # Cd 2.689 on its uniform D3Q19 grid
+    assert abs(cd / REF_CD_RE20 - 1.0) < 0.005
 
 
 @pytest.mark.parametrize("re,d,passes,tol", [(100.0, 48, 30.0, 0.03), (300.0, 48, 40.0, 0.06)])
@@ -128,6 +135,8 @@ def test_rotating_sphere_stokes_torque():
     _say(f"  torque {-t:.4e} vs 8 pi mu R^3 omega {stokes:.4e} ({(-t / stokes - 1) * 100:+.2f} %), "
          f"thrust {r.summary.thrust_n:.2e}, {r.summary.steps} steps, {r.summary.seconds:.1f} s")
     assert abs(-t / stokes - 1.0) < 0.05
+    # the reference's own result (docs/VALIDATION.md:102): 1.146e-3 N m
+    assert abs(-t / REF_SPIN_TORQUE - 1.0) < 0.005
 
 
 def test_prop_in_forward_flight_loses_thrust():
