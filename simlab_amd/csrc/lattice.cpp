@@ -695,10 +695,36 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     return SIMLAB_OK;
 }
 
-int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, double *ft, bool still) {
+// Where a run keeps the per-block force partials and the per-chunk force
+// history: the lattice's own buffers (kmax = KMAX steps per chunk), or an
+// array the caller provides with room for kmax steps of both.
+struct ForceScratch {
+    double *partials = nullptr;   // [kmax][blocks][8]
+    double *hist = nullptr;       // [kmax][HIST_STRIDE]
+    size_t kmax = KMAX;
+};
+
+int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, double *ft, bool still,
+              const ForceScratch *fs = nullptr) {
     if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
     if (!still && nsteps && !steps) return fail(SIMLAB_ERR_INVALID, "null steps");
     hip_check(hipSetDevice(lat->device), "hipSetDevice");
+    // the kernels read the partials and history pointers from the lattice's
+    // device view: point it at the given scratch for this call only
+    struct ViewRestore {
+        DevLattice &L;
+        double *partials, *hist;
+        ~ViewRestore() {
+            L.partials = partials;
+            L.hist = hist;
+        }
+    } restore{lat->L, lat->L.partials, lat->L.hist};
+    size_t kmax = KMAX;
+    if (fs) {
+        lat->L.partials = fs->partials;
+        lat->L.hist = fs->hist;
+        kmax = fs->kmax;
+    }
     simlab_step_in idn{};
     for (int k = 0; k < 3; ++k) {
         idn.rot[k][k] = 1.0;
@@ -723,7 +749,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
     lat->last_main_launches = 0;
     size_t done = 0;
     while (done < nsteps) {
-        const size_t chunk = std::min<size_t>(KMAX, nsteps - done);
+        const size_t chunk = std::min<size_t>(kmax, nsteps - done);
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX], st), "event");
         for (size_t k = 0; k < chunk; ++k) {
             const simlab_step_in &s = still ? idn : steps[done + k];

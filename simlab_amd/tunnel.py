@@ -600,12 +600,58 @@ def _window_check(w: np.ndarray, spin: bool, axis_l: np.ndarray):
     return worst, se_rel, trend, half
 
 
+class Cancelled(Exception):
+    """A run stopped by its caller (core progress.rs:44-50, "stopped")."""
+
+    def __init__(self):
+        super().__init__("stopped")
+
+
+class Progress:
+    """What a run reports to and asks of its caller (core progress.rs:6-13):
+    `report(fraction, message)` after every batch, `metric(name, step,
+    value)` for the live plots (thrust_n / torque_nm of a spinning part,
+    drag_n / lift_n in a wind), and `cancelled()`, asked after the
+    voxelization and after every batch; a true answer ends the run with
+    `Cancelled`. The base class reports nothing and never cancels."""
+
+    def report(self, fraction: float, message: str) -> None:
+        pass
+
+    def metric(self, name: str, step: int, value: float) -> None:
+        pass
+
+    def cancelled(self) -> bool:
+        return False
+
+
+class _CallbackProgress(Progress):
+    """A plain `progress(fraction, message)` callable as a Progress."""
+
+    def __init__(self, fn):
+        self.fn = fn
+
+    def report(self, fraction, message):
+        self.fn(fraction, message)
+
+
+def _as_progress(progress) -> Progress:
+    if progress is None:
+        return Progress()
+    if isinstance(progress, Progress) or hasattr(progress, "cancelled"):
+        return progress
+    return _CallbackProgress(progress)
+
+
 def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[SolverOptions] = None,
              progress=None) -> WindResult:
     """Runs the wind tunnel (tunnel.rs:902-1392). `surface` in metres;
     `unit_m` metres per model unit, used for the spin centre and the slice
-    positions. `progress(fraction, message)` is called after every batch."""
+    positions. `progress` is a Progress (or a plain `progress(fraction,
+    message)` callable): reports and live metrics after every batch; a run
+    it cancels raises `Cancelled` (tunnel.rs:947-949, 1139-1141)."""
     t_start = time.perf_counter()
+    progress = _as_progress(progress)
     opt = opt or SolverOptions()
     p = plan(cfg, surface, unit_m, opt, None)
     warnings: List[str] = []
@@ -624,6 +670,8 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         vox = voxelize_static(np.asarray(p.center) + to_body, surface.triangles, p.dims)
         _check_thin(warnings, float(len(vox.solid)), surface, p.dx)
         geom = Geometry.static(vox)
+    if progress.cancelled():
+        raise Cancelled()
     flip = -1.0 if volume(surface) < 0.0 else 1.0
     normals = (_vertex_normals(surface) * flip) @ p.r.T
     nx, ny, nz = p.dims
@@ -701,11 +749,18 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
             f_model = U.to_model(p.r, fm[:3]) * fscale
             m_model = U.to_model(p.r, fm[3:]) * mscale
             history.append((step / unit_steps, f_model, m_model))
-            if progress is not None:
-                el = time.perf_counter() - t_loop
-                progress(0.02 + 0.96 * step / total_steps,
-                         f"Solving: {step} of {total_steps} steps ({step / unit_steps:.1f} "
-                         f"{'revolutions' if p.still else 'flow passes'}, {step * p.updates_per_step / el / 1e6:.0f} MLUPs)")
+            if p.spin:
+                progress.metric("thrust_n", step, float(np.dot(f_model, p.axis_model)))
+                progress.metric("torque_nm", step, float(np.dot(m_model, p.axis_model)))
+            if not p.still:
+                progress.metric("drag_n", step, float(np.dot(f_model, p.r[0])))
+                progress.metric("lift_n", step, float(np.dot(f_model, p.r[2])))
+            el = time.perf_counter() - t_loop
+            progress.report(0.02 + 0.96 * step / total_steps,
+                            f"Solving: {step} of {total_steps} steps ({step / unit_steps:.1f} "
+                            f"{'revolutions' if p.still else 'flow passes'}, {step * p.updates_per_step / el / 1e6:.0f} MLUPs)")
+            if progress.cancelled():
+                raise Cancelled()
             if not np.all(np.isfinite(hist[:step])):
                 raise RuntimeError("the flow solution blew up (unstable); try a finer grid or a lower speed")
             if step > w0 + min_win:

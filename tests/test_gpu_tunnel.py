@@ -217,3 +217,28 @@ def test_sphere_study_in_fp16_storage():
     assert r16.summary.storage == Storage.FP16
     assert any("FP16 storage" in w for w in r16.summary.warnings)
     assert abs(cd16 / cd32 - 1.0) < 0.005
+
+
+def test_cancel_after_the_first_batch():
+    # tunnel.rs:1139-1141: asked after every batch, after the report and the
+    # live metrics (drag_n, lift_n in a wind)
+    class StopAfterOne(T.Progress):
+        def __init__(self):
+            self.reports, self.metrics = [], []
+
+        def report(self, fraction, message):
+            self.reports.append((fraction, message))
+
+        def metric(self, name, step, value):
+            self.metrics.append((name, step, value))
+
+        def cancelled(self):
+            return len(self.reports) >= 1
+
+    pr = StopAfterOne()
+    cfg, m = T.sphere_case(100.0, 16)
+    with pytest.raises(T.Cancelled):
+        T.run_with(cfg, m, 1e-3, None, pr)
+    assert len(pr.reports) == 1 and pr.reports[0][1].startswith("Solving: ")
+    assert [n for n, _, _ in pr.metrics] == ["drag_n", "lift_n"]
+    assert pr.metrics[0][2] > 0.0

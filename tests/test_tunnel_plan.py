@@ -222,3 +222,19 @@ def test_layout_limit_is_the_lattices():
     cfg, m = T.sphere_case(1000.0, 128)
     p = T.plan(cfg, m, 1e-3, T.SolverOptions())
     assert int(np.prod(p.dims)) > 537_000_000 and p.storage == Storage.FP32
+
+
+def test_a_cancelled_run_stops_before_the_lattice():
+    # tunnel.rs:947-949: cancellation is asked right after the voxelization,
+    # before any GPU work; the error reads "stopped" (core progress.rs:44-50)
+    class Stop(T.Progress):
+        asked = 0
+
+        def cancelled(self):
+            Stop.asked += 1
+            return True
+
+    cfg, m = T.sphere_case(100.0, 8)
+    with pytest.raises(T.Cancelled, match="stopped"):
+        T.run_with(cfg, m, 1e-3, None, Stop())
+    assert Stop.asked == 1
