@@ -201,6 +201,22 @@ int simlab_cfd_lattice_run(simlab_lattice *lat, const simlab_step_in *steps,
 int simlab_cfd_lattice_run_still(simlab_lattice *lat, size_t nsteps,
                                  double *ft_out);
 
+/* `run` that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation). The same steps
+ * and results as simlab_cfd_lattice_run, but the kernels' per-block force
+ * partials and the per-chunk force history live in a 16 MiB device array of
+ * the library (one per GPU, held until the process ends) instead of the
+ * lattice's own buffers. A chunk then holds at most
+ * 16 MiB / ((8 * wall blocks + 8) * 8 B) steps (wall blocks = the lattice's
+ * wall-record cells / 256 plus its halfway wall cells / 256, each rounded
+ * up); a lattice whose single step needs more than 16 MiB fails with
+ * SIMLAB_ERR_INVALID. Every call on one GPU shares that array and leaves
+ * the last call's contents in it: calls that overlap on one GPU (from
+ * several threads) race. Timing is not recorded. */
+int simlab_cfd_lattice_run_noalloc(simlab_lattice *lat, const simlab_step_in *steps,
+                                   size_t nsteps, double *ft_out);
+
 /* `Lattice::macros` (lattice.rs:818-835): (rho, ux, uy, uz) of every cell,
  * NaN where solid or on a face; needs debug_dump. out: cells x 4 floats. */
 int simlab_cfd_lattice_macros(simlab_lattice *lat, float *out);

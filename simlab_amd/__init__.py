@@ -117,6 +117,7 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_lattice_mass": (I, [V, C.POINTER(D)]),
         "simlab_cfd_lattice_get_pops": (I, [V, P, C.c_size_t]),
         "simlab_cfd_lattice_set_pops": (I, [V, P, C.c_size_t]),
+        "simlab_cfd_lattice_run_noalloc": (I, [V, C.POINTER(_StepIn), C.c_size_t, P]),
         "simlab_cfd_lattice_set_timing": (I, [V, I]),
         "simlab_cfd_lattice_timing": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64)]),
         "simlab_cfd_lattice_autotune": (I, [V, C.c_uint32, I, P, C.POINTER(D)]),
@@ -525,6 +526,14 @@ class Lattice:
         out = np.zeros((len(steps), 6), np.float64)
         arr = steps_to_c(steps)
         _check(self._lib, self._lib.simlab_cfd_lattice_run(self._h, arr, len(steps), _ptr(out)))
+        return out
+
+    def run_noalloc(self, steps: Sequence[StepIn]) -> np.ndarray:
+        """`run` whose force partials live in the library's device scratch
+        array: no device allocation (simlab_cfd_lattice_run_noalloc)."""
+        out = np.zeros((len(steps), 6), np.float64)
+        arr = steps_to_c(steps)
+        _check(self._lib, self._lib.simlab_cfd_lattice_run_noalloc(self._h, arr, len(steps), _ptr(out)))
         return out
 
     def run_still(self, nsteps: int) -> np.ndarray:

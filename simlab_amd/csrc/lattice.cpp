@@ -820,7 +820,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
         }
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
         hip_check(launch_reduce_hist(lat->L, (uint32_t)chunk, st), "force reduction kernel");
-        hip_check(hipMemcpyAsync(lat->hist_host, lat->hist.p, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st), "force copy");
+        hip_check(hipMemcpyAsync(lat->hist_host, lat->L.hist, chunk * HIST_STRIDE * 8, hipMemcpyDeviceToHost, st), "force copy");
         hip_check(hipStreamSynchronize(st), "GPU wait failed");
         if (lat->timing) {
             float ms = 0.0f;
@@ -933,6 +933,70 @@ int simlab_cfd_lattice_run_still(simlab_lattice *lat, size_t nsteps, double *ft_
     return guarded([&]() -> int {
         try {
             return run_steps(lat, nullptr, nsteps, ft_out, true);
+        } catch (const HipError &e) {
+            return fail(e.code, e.what());
+        }
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free run
+namespace {
+// The force partials and history of simlab_cfd_lattice_run_noalloc: a
+// file-scope device array (each GPU the library runs on holds its own copy
+// until the process ends). Host code never uses its name as an address: the
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device on the entry point's first call there.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// the device's address of `scratch`; the device must be current
+double *scratch_on(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) throw HipError(SIMLAB_ERR_GPU, "run_noalloc: device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int simlab_cfd_lattice_run_noalloc(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, double *ft_out) {
+    return guarded([&]() -> int {
+        if (!lat) return fail(SIMLAB_ERR_INVALID, "null lattice");
+        try {
+            hip_check(hipSetDevice(lat->device), "hipSetDevice");
+            // per step: one 8-double partial per wall block, one history row
+            const size_t blocks = (size_t)lat->L.wall_blocks + lat->L.hw_blocks;
+            const size_t per_step = (blocks * 8 + HIST_STRIDE) * sizeof(double);
+            const size_t kmax = std::min<size_t>(KMAX, NOALLOC_SCRATCH_BYTES / per_step);
+            if (kmax == 0)
+                return fail(SIMLAB_ERR_INVALID, "run_noalloc: the force partials of one step (" +
+                                                    std::to_string(per_step) + " bytes for " + std::to_string(blocks) +
+                                                    " wall blocks) exceed the " +
+                                                    std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB device scratch");
+            double *base = scratch_on(lat->device);
+            ForceScratch fs;
+            fs.partials = base;
+            fs.hist = base + kmax * blocks * 8;
+            fs.kmax = kmax;
+            // timing would create events on first use: off for this call
+            struct TimingOff {
+                simlab_lattice *l;
+                bool was;
+                ~TimingOff() { l->timing = was; }
+            } off{lat, lat->timing};
+            lat->timing = false;
+            return run_steps(lat, steps, nsteps, ft_out, false, &fs);
         } catch (const HipError &e) {
             return fail(e.code, e.what());
         }
