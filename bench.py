@@ -77,14 +77,15 @@ def timed_run(sa, W, storage, steps, warmup):
     return res, n
 
 
-def large_runs(sa, W, steps, warmup):
+def large_runs(sa, W, steps, warmup, cases=("sphere", "prop"), storages=("fp16", "fp32")):
     """The 384^3 cases: the sphere channel scaled to 384^3 (sphere r=24 at
     x=96, Re 1000, Cs 0.1) and the spinning two-blade prop (radius 120,
     tip speed 0.08, axial inflow 0.01, Cs 0.16), FP16 and FP32 storage;
     MLUPS over all cells from the wall clock, the bulk kernel's algorithmic
     GB/s from its HIP events."""
     out = {}
-    for name, st in (("fp16", sa.Storage.FP16), ("fp32", sa.Storage.FP32)):
+    sts = [(n, {"fp16": sa.Storage.FP16, "fp32": sa.Storage.FP32}[n]) for n in storages]
+    for name, st in (sts if "sphere" in cases else []):
         print(f"# 384^3 sphere channel {name}", file=sys.stderr, flush=True)
         setup, geom, meta = W.sphere_channel((384, 384, 384), 24.0, 96.0, 1000.0, 0.05, 0.1, st)
         lat = sa.Lattice(setup, geom)
@@ -103,7 +104,7 @@ def large_runs(sa, W, steps, warmup):
             "kernel_gbs": round(gbs, 1), "kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
             "tile": list(lat.tile())}
         del lat
-    for name, st in (("fp16", sa.Storage.FP16), ("fp32", sa.Storage.FP32)):
+    for name, st in (sts if "prop" in cases else []):
         print(f"# 384^3 spinning prop {name}", file=sys.stderr, flush=True)
         setup, geom, meta = W.spinning_prop(storage=st)
         lat = sa.Lattice(setup, geom)
@@ -205,6 +206,45 @@ def study_arm(sa, W, steps, batch_steps):
     return out
 
 
+def study_batch(sa, W, steps, sizes=(128, 192, 256, 384)):
+    """The drone arm at several sizes in FP16, autotuned: MLUPS over all the
+    members' cells stepped one lattice after the other, as one batch with one
+    kernel per step (the default), and as a batch with a special and a bulk
+    kernel on two streams (members over 4 Mi cells stepping alone)."""
+    lats, cells, tiles = [], 0, {}
+    for n in sizes:
+        setup, geom, meta = W.drone_arm(dims=(n, n, n), storage=sa.Storage.FP16, scale=n / 256.0)
+        lat = sa.Lattice(setup, geom)
+        t, us = lat.autotune(steps=20, use_cache=True)
+        tiles[str(n)] = {"tile": list(t), "step_us": round(us, 1)}
+        lats.append(lat)
+        cells += setup.cells()
+    out = {"sizes": list(sizes), "tiles": tiles, "steps": steps}
+    for lat in lats:
+        lat.run_still(20)
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for lat in lats:
+            lat.run_still(steps)
+        reps.append(cells * steps / (time.perf_counter() - t0) / 1e6)
+    out["sequential_mlups"] = round(max(reps), 1)
+    for mode in ("fused", "split"):
+        os.environ["SIMLAB_CFD_BATCH"] = mode
+        b = sa.Batch(lats)
+        b.run_still(20)
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            b.run_still(steps)
+            reps.append(cells * steps / (time.perf_counter() - t0) / 1e6)
+        out[f"batched_{mode}_mlups"] = round(max(reps), 1)
+        del b
+    os.environ.pop("SIMLAB_CFD_BATCH", None)
+    out["best_of"] = 3
+    return out
+
+
 def drag_run(sa, W, storage, steps):
     setup, geom, meta = W.sphere_channel_large(storage)
     lat = sa.Lattice(setup, geom)
@@ -303,14 +343,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--no-384", action="store_true", help="skip the 384^3 sphere-channel and spinning-prop timings")
-    ap.add_argument("--study", choices=["prop", "arm"], default=None,
+    ap.add_argument("--study", choices=["prop", "arm", "batch"], default=None,
                     help="accuracy studies instead of the benchmark line: the 384^3 spinning prop's thrust and "
                          "torque per storage (prop), or the 512x256x256 drone arm's drag per storage plus the "
                          "batched arm at four sizes (arm); one JSON object")
     ap.add_argument("--study-steps", type=int, default=0, help="steps of a study run (prop 20000, arm 10000)")
     ap.add_argument("--only-384", action="store_true",
                     help="only the 384^3 timings (one JSON object of them; for profiling those kernels alone)")
-    ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16)")
+    ap.add_argument("--only", default="", help="comma list of storages to time (fp16,fp32,bf16); with --only-384 "
+                                                  "fp16,fp32")
+    ap.add_argument("--case", default="sphere,prop", help="with --only-384: comma list of the 384^3 cases to time "
+                                                         "(sphere, prop)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile into bytes)")
@@ -332,11 +375,16 @@ def main():
     if args.study == "prop":
         print(json.dumps({"study_prop384": study_prop(sa, W, args.study_steps or 20000)}), flush=True)
         return
+    if args.study == "batch":
+        print(json.dumps({"study_batch": study_batch(sa, W, args.study_steps or 300)}), flush=True)
+        return
     if args.study == "arm":
         print(json.dumps({"study_arm": study_arm(sa, W, args.study_steps or 10000, 500)}), flush=True)
         return
     if args.only_384:
-        print(json.dumps({"n384": large_runs(sa, W, args.steps, args.warmup)}), flush=True)
+        cases = tuple(c for c in args.case.split(",") if c)
+        sts = tuple(c for c in (args.only.split(",") if args.only else ["fp16", "fp32"]) if c)
+        print(json.dumps({"n384": large_runs(sa, W, args.steps, args.warmup, cases, sts)}), flush=True)
         return
     storages = [s for s in (args.only.split(",") if args.only else ["fp16", "fp32", "bf16"]) if s]
     results = {}

@@ -125,20 +125,28 @@ struct StepArgs {
     float omega[4];
     uint32_t seed;
     uint32_t slot;
+    // a spinning part: the rotation the lattice's previous voxelization used
+    // (vq_valid = 0 when there is none, or it is not known)
+    float vq[3][4];
+    uint32_t vq_valid;
 };
 
 // A batch of lattices stepped together (device arrays prepared by the host):
 // lats[nl] device views, boff_bulk[nl + 1] / boff_sp[nl + 1] first block of
 // each lattice in the bulk / special launch, `threads` (SPECIAL_BLOCK) per
-// bulk block (each lattice's bx * by divides it).
+// bulk block (each lattice's bx * by divides it). fused: one kernel per step
+// on s_special (the special blocks, then the bulk blocks at boff_bulk after
+// them; boff_bulk must follow boff_sp in the same array).
 struct BatchLaunch {
     const DevLattice *lats;
     const uint32_t *boff_bulk, *boff_sp;
     uint32_t nl, nbulk, nsp, threads;
     int32_t storage;
     bool force, sponge;
+    bool fused;
 };
-// special kernel of the batch on s_special, bulk kernel on s_bulk
+// special kernel of the batch on s_special, bulk kernel on s_bulk (or the
+// one-kernel step on s_special)
 hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk);
 
 // The launchers of one population storage (kernels.hip compiled once per
@@ -158,6 +166,9 @@ struct StorageLaunchers {
                                hipStream_t);
     hipError_t (*batch_step)(const BatchLaunch &, int, const StepArgs &, hipStream_t, hipStream_t);
     hipError_t (*voxelize)(const DevLattice &, int, const StepArgs &, hipStream_t);
+    hipError_t (*special)(const DevLattice &, int, const StepArgs &, hipStream_t);
+    hipError_t (*bulk_z)(const DevLattice &, int, const StepArgs &, hipStream_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                         hipEvent_t, hipEvent_t);
 };
 extern const StorageLaunchers launchers_fp32, launchers_fp16, launchers_bf16, launchers_fp64;
 
@@ -175,6 +186,12 @@ hipError_t launch_stream_collide(const DevLattice &L, int parity, const StepArgs
 // SPECIAL_BLOCK); ev0/ev1 as above, for that kernel
 hipError_t launch_step_fused(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s, hipEvent_t ev0 = nullptr,
                              hipEvent_t ev1 = nullptr);
+// the special kernel alone, and the bulk kernel over nplanes z-planes:
+// z = zbase + k, and z += zskip_len from zskip_at on (k = 0 .. nplanes-1)
+hipError_t launch_special(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
+hipError_t launch_bulk_planes(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s, uint32_t zbase,
+                              uint32_t nplanes, uint32_t zskip_at, uint32_t zskip_len, hipEvent_t ev0 = nullptr,
+                              hipEvent_t ev1 = nullptr);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
 // re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)

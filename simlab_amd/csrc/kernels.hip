@@ -1045,7 +1045,12 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
             sx = x;
             sy = y;
             sz = z;
-        } else {
+        } else if (fg != 0u) {
+            // a solid cell of a spinning part (the momentum it took when
+            // covered), or a record another cell holds. A plain fluid cell
+            // (flags 0: a box cell far from the part) has F = T = 0: its
+            // record was reset when it left the part's neighbourhood and is
+            // not written while it is far
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 F[k] = rft[k * NR + r];
@@ -1162,10 +1167,17 @@ template <class C, int Q> __global__ void __launch_bounds__(256) k_xfaces(DevLat
 template <class C, int P, bool FORCE, bool SPONGE> constexpr int bulk_waves() {
     return ((std::is_same_v<C, CF16> || std::is_same_v<C, CBF16>) && P == 1 && !FORCE && SPONGE) ? 5 : 1;
 }
+// z-planes of a bulk launch: z = zbase + blockIdx.y, and z += zskip_len from
+// zskip_at on (all planes: {0, nz, 0})
+struct ZMap {
+    uint32_t zbase, zskip_at, zskip_len;
+};
 template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(bulk_waves<C, P, FORCE, SPONGE>())))
-k_bulk(DevLattice L, StepArgs a) {
-    bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, blockIdx.y, threadIdx.y * blockDim.x + threadIdx.x);
+k_bulk(DevLattice L, StepArgs a, ZMap zm) {
+    uint32_t z = zm.zbase + blockIdx.y;
+    if (z >= zm.zskip_at) z += zm.zskip_len;
+    bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, z, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
 // One launch per step, one stream: blocks [0, wall_blocks + hw_blocks) do
@@ -1230,6 +1242,35 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_special_batch(const DevLattic
     __shared__ double sh[16][6];
     const uint32_t i = batch_member(boff, nl, blockIdx.x);
     special_block<C, P, FORCE, SPONGE>(lats[i], a, blockIdx.x - boff[i], sh);
+}
+
+// One launch per step for a whole batch: blocks [0, boff[nl]) are the
+// members' special blocks (member i from boff[i]), the rest their bulk
+// tiles (member i from bulk[i], bulk = boff + nl + 1, counted after the
+// special blocks), each member at its own tile (bx * by divides
+// SPECIAL_BLOCK). No second stream and no events per step; the host orders
+// the bulk ranges largest member first, so the small members' tiles fill the
+// large member's tail.
+template <class C, int P, bool FORCE, bool SPONGE>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE>())))
+k_step_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ boff, uint32_t nl, StepArgs a) {
+    __shared__ double sh[16][6];
+    const uint32_t nsp = boff[nl];
+    if (blockIdx.x < nsp) {
+        const uint32_t i = batch_member(boff, nl, blockIdx.x);
+        special_block<C, P, FORCE, SPONGE>(lats[i], a, blockIdx.x - boff[i], sh);
+        return;
+    }
+    const uint32_t *bulk = boff + nl + 1;
+    const uint32_t b = blockIdx.x - nsp;
+    const uint32_t i = batch_member(bulk, nl, b);
+    const DevLattice &L = lats[i];
+    const uint32_t local = __builtin_amdgcn_readfirstlane(b - bulk[i]);
+    const uint32_t tt = L.bx * L.by;
+    const uint32_t g = local * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
+    const uint32_t tiles = L.xblocks * L.yblocks;
+    if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
+    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 // ---------------------------------------------------------------- moving geometry
@@ -1303,32 +1344,20 @@ __device__ __forceinline__ void box_cell(const DevLattice &L, uint32_t k, int &p
     pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
 }
 
-// Moller-Trumbore for the segment o + t d, t in [0, 1]; 9 for no hit
-__device__ __forceinline__ float ray_tri(V3f o, V3f d, V3f v0, V3f e1, V3f e2) {
-#pragma clang fp contract(off)
-    const V3f pv = cross3(d, e2);
-    const float det = dot3(e1, pv);
-    if (__builtin_fabsf(det) < 1e-12f) return 9.0f;
-    const float inv = 1.0f / det;
-    const V3f tv = o - v0;
-    const float u = dot3(tv, pv) * inv;
-    if (u < 0.0f || u > 1.0f) return 9.0f;
-    const V3f qv = cross3(tv, e1);
-    const float v = dot3(d, qv) * inv;
-    if (v < 0.0f || u + v > 1.0f) return 9.0f;
-    const float t = dot3(e2, qv) * inv;
-    if (t < -1e-4f || t > 1.0f) return 9.0f;
-    // max(t, 0), with +0 for t = -0: the hit bits are ordered as unsigned
-    // integers, which sort non-negative floats only
-    return t > 0.0f ? t : 0.0f;
-}
-
 #if SIMLAB_KSTORE < 0
 // one wave per triangle; its lanes share the lattice cells of the
-// triangle's bounding box (grown by 1.415)
+// triangle's bounding box (grown by 1.415). Moller-Trumbore for the link
+// segment p + s c_i, s in [0, 1] (shader.rs ray_tri), split into what
+// depends on the triangle and the direction only (pv = c_i x e2, e1 . pv and
+// its reciprocal, n . c_i: once per wave) and what depends on the cell
+// (tv = p - v0, tv x e1 and its dot with e2: once per cell), each the same
+// float operation on the same operands in the same order as the test
+// written out per link, so the hits and q bits are those of the reference's
+// arithmetic.
 __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
 #pragma clang fp contract(off)
-    const uint32_t t = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
+    const uint32_t lane = threadIdx.x % 64;
     if (t >= L.ntri) return;
     const float4 t0 = L.tris[3 * t], t1 = L.tris[3 * t + 1], t2 = L.tris[3 * t + 2];
     const V3f A = centerf(L) + from_body(a.r, v3(t0.x, t0.y, t0.z));
@@ -1350,26 +1379,56 @@ __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
         }
     }
     if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return;
+    // per odd direction i = 2j + 1: n . c_i, pv = c_i x e2, 1 / (e1 . pv)
+    float dn[9], inv[9];
+    V3f pv[9];
+    uint32_t okd = 0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int i = 2 * j + 1;
+        const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
+        dn[j] = dot3(nn, ci);
+        pv[j] = cross3(ci, e2);
+        const float det = dot3(e1, pv[j]);
+        inv[j] = 1.0f;
+        if (!(__builtin_fabsf(det) < 1e-12f)) {
+            inv[j] = 1.0f / det;
+            okd |= 1u << j;
+        }
+    }
     const uint32_t ex = (uint32_t)(hi[0] - lo[0] + 1), ey = (uint32_t)(hi[1] - lo[1] + 1), ez = (uint32_t)(hi[2] - lo[2] + 1);
     for (uint32_t c = lane; c < ex * ey * ez; c += 64) {
         const int x = lo[0] + (int)(c % ex), y = lo[1] + (int)((c / ex) % ey), z = lo[2] + (int)(c / (ex * ey));
         const V3f p = v3((float)x, (float)y, (float)z);
-        const float d0 = dot3(nn, p - A);
+        const V3f tv = p - A;
+        const float d0 = dot3(nn, tv);
         if (__builtin_fabsf(d0) > 1.415f) continue;
         const uint32_t k = ((uint32_t)(z - L.box_lo[2]) * L.box_n[1] + (uint32_t)(y - L.box_lo[1])) * L.box_n[0] +
                            (uint32_t)(x - L.box_lo[0]);
-        for (int i = 1; i < 19; i += 2) {
-            const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
-            const float d1 = d0 + dot3(nn, ci);
+        const V3f qv = cross3(tv, e1);
+        const float tq = dot3(e2, qv);
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int i = 2 * j + 1;
+            const float d1 = d0 + dn[j];
             if (d0 * d1 > 0.0f && __builtin_fabsf(d0) > 1e-6f) continue;
-            const float tt = ray_tri(p, ci, A, e1, e2);
-            if (tt <= 1.0f) {
-                uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
-                const uint32_t tb = __float_as_uint(tt);   // tt >= +0 (ray_tri)
-                atomicMin(o, tb);
-                atomicMax(o + 1, tb);
-                atomicAdd(o + 2, 1u);
-            }
+            if (!(okd & (1u << j))) continue;
+            // ray_tri for the segment p + s c_i, s in [0, 1]
+            const float u = dot3(tv, pv[j]) * inv[j];
+            if (u < 0.0f || u > 1.0f) continue;
+            const V3f ci = v3((float)rcx(i), (float)rcy(i), (float)rcz(i));
+            const float v = dot3(ci, qv) * inv[j];
+            if (v < 0.0f || u + v > 1.0f) continue;
+            float tt = tq * inv[j];
+            if (tt < -1e-4f || tt > 1.0f) continue;
+            // max(t, 0), with +0 for t = -0: the hit bits are ordered as
+            // unsigned integers, which sort non-negative floats only
+            tt = tt > 0.0f ? tt : 0.0f;
+            uint32_t *o = L.qscr + ((size_t)k * 9 + j) * 3;
+            const uint32_t tb = __float_as_uint(tt);
+            atomicMin(o, tb);
+            atomicMax(o + 1, tb);
+            atomicAdd(o + 2, 1u);
         }
     }
 }
@@ -1445,12 +1504,23 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
         // hit; the outcome of the full rule below is known. Farther still
         // (outside plane 1) the cell steps as a plain fluid cell in the
         // bulk kernel and its record is left as it is.
+        const bool in1 = near_part(L, bnow, 1);
+        if (a.vq_valid) {
+            // far at the previous voxelization too: its links, q and F, T
+            // were reset then (or at an earlier one) and nothing has written
+            // them since; only the flag may change (plane 1)
+            const V3f bprev = to_body(a.vq, pc);
+            if (!near_part(L, bprev)) {
+                if (in1 != near_part(L, bprev, 1)) L.flags[n] = in1 ? (k + 1u) << 2 : 0u;
+                return;
+            }
+        }
 #pragma unroll
         for (int w = 0; w < 9; ++w) L.rq[(size_t)w * NR + k] = 0x3C003C00u;
         L.rmask[k] = 0u;
 #pragma unroll
         for (int j = 0; j < 6; ++j) rft[j * NR + k] = (T)0;
-        L.flags[n] = near_part(L, bnow, 1) ? (k + 1u) << 2 : 0u;
+        L.flags[n] = in1 ? (k + 1u) << 2 : 0u;
         return;
     }
     const bool solid_now = solid_at(L, a, px, py, pz, false);
@@ -2070,8 +2140,32 @@ hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_sp
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_timed(k_bulk<C, P, FORCE, SPONGE>, dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), s_bulk, ev0, ev1,
-                        L, a);
+                        L, a, ZMap{0u, L.nz, 0u});
 }
+
+template <class C, int P, bool FORCE, bool SPONGE>
+hipError_t special_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s) {
+    k_special<C, P, FORCE, SPONGE><<<L.wall_blocks + L.hw_blocks, SPECIAL_BLOCK, 0, s>>>(L, a);
+    return hipGetLastError();
+}
+
+template <class C, int P, bool FORCE, bool SPONGE>
+hipError_t bulk_z_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s, uint32_t nplanes, const ZMap &zm,
+                          hipEvent_t ev0, hipEvent_t ev1) {
+    if (nplanes == 0) return hipSuccess;
+    return launch_timed(k_bulk<C, P, FORCE, SPONGE>, dim3(L.xblocks * L.yblocks, nplanes), dim3(L.bx, L.by), s, ev0, ev1,
+                        L, a, zm);
+}
+
+// the (FORCE, SPONGE) variant of a launch template for this lattice
+#define SIMLAB_VARIANT(FN, C, P, ...)                                              \
+    do {                                                                           \
+        const bool force_ = L.forcing != 0, sponge_ = L.sponge != 0;               \
+        if (!force_ && !sponge_) return FN<C, P, false, false>(__VA_ARGS__);       \
+        if (force_ && !sponge_) return FN<C, P, true, false>(__VA_ARGS__);        \
+        if (!force_ && sponge_) return FN<C, P, false, true>(__VA_ARGS__);        \
+        return FN<C, P, true, true>(__VA_ARGS__);                                  \
+    } while (0)
 
 template <class C, int P, bool FORCE, bool SPONGE>
 hipError_t fused_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
@@ -2162,6 +2256,18 @@ hipError_t t_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipSt
     return step_launch_p<KC>(L, p, a, s1, s2, ev0, ev1);
 }
 
+hipError_t t_special(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    if (p == 0) SIMLAB_VARIANT(special_launch4, KC, 0, L, a, s);
+    SIMLAB_VARIANT(special_launch4, KC, 1, L, a, s);
+}
+
+hipError_t t_bulk_z(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, uint32_t zbase, uint32_t nplanes,
+                    uint32_t zskip_at, uint32_t zskip_len, hipEvent_t ev0, hipEvent_t ev1) {
+    const ZMap zm{zbase, zskip_at, zskip_len};
+    if (p == 0) SIMLAB_VARIANT(bulk_z_launch4, KC, 0, L, a, s, nplanes, zm, ev0, ev1);
+    SIMLAB_VARIANT(bulk_z_launch4, KC, 1, L, a, s, nplanes, zm, ev0, ev1);
+}
+
 hipError_t t_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     if (L.bx * L.by > SPECIAL_BLOCK || SPECIAL_BLOCK % (L.bx * L.by) != 0) return hipErrorInvalidConfiguration;
     return fused_launch_p<KC>(L, p, a, s, ev0, ev1);
@@ -2226,6 +2332,10 @@ hipError_t t_adjoint_step(const DevLattice &L, const DevLattice &A, int p, const
 
 template <int P, bool FORCE, bool SPONGE>
 hipError_t batch_launch4(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    if (b.fused) {
+        k_step_batch<KC, P, FORCE, SPONGE><<<b.nsp + b.nbulk, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
+        return hipGetLastError();
+    }
     k_special_batch<KC, P, FORCE, SPONGE><<<b.nsp, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -2263,7 +2373,7 @@ hipError_t t_voxelize(const DevLattice &L, int p, const StepArgs &a, hipStream_t
 extern const StorageLaunchers KTABLE;
 const StorageLaunchers KTABLE = {t_init,       t_faces,          t_xfaces,   t_stream_collide, t_step_fused,
                                  t_macros,     t_mass_partial,   t_samplers, t_pbb_step,       t_adjoint_step,
-                                 t_batch_step, t_voxelize};
+                                 t_batch_step, t_voxelize,       t_special,  t_bulk_z};
 #endif
 #undef KTABLE
 
@@ -2291,6 +2401,13 @@ hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, 
 }
 hipError_t launch_step_fused(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     return kl(L.storage).step_fused(L, p, a, s, ev0, ev1);
+}
+hipError_t launch_special(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    return kl(L.storage).special(L, p, a, s);
+}
+hipError_t launch_bulk_planes(const DevLattice &L, int p, const StepArgs &a, hipStream_t s, uint32_t zbase, uint32_t nplanes,
+                              uint32_t zskip_at, uint32_t zskip_len, hipEvent_t ev0, hipEvent_t ev1) {
+    return kl(L.storage).bulk_z(L, p, a, s, zbase, nplanes, zskip_at, zskip_len, ev0, ev1);
 }
 hipError_t launch_macros(const DevLattice &L, int p, float4 *out, hipStream_t s) { return kl(L.storage).macros(L, p, out, s); }
 hipError_t launch_mass(const DevLattice &L, int p, double *partials, uint32_t nblocks, double *out, hipStream_t s) {

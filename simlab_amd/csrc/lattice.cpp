@@ -122,6 +122,10 @@ struct simlab_lattice {
     hipEvent_t ev_vox = nullptr, ev_scat = nullptr;
     bool scat_ready = false;         // the next step's hits are on stream3 behind ev_scat
     bool spin_overlap = true;
+    // the rotation of the last voxelization (columns as in StepArgs), valid
+    // while the records it wrote are the lattice's state
+    float vox_prev[3][4] = {};
+    bool vox_prev_valid = false;
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
     DevBuf xsrc;                     // deferred x-face sources (L.xsrc)
@@ -337,6 +341,8 @@ void pack_step(const simlab_step_in &s, uint64_t step, uint32_t slot, StepArgs &
     a.omega[3] = 0.0f;
     a.seed = (uint32_t)step * 2654435761u;
     a.slot = slot;
+    std::memset(a.vq, 0, sizeof a.vq);
+    a.vq_valid = 0;
 }
 
 int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const simlab_probes *pr, simlab_lattice **out) {
@@ -757,6 +763,11 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             const int par = (int)(step % 2);
             StepArgs a;
             pack_step(s, step, (uint32_t)k, a);
+            const uint32_t tt = lat->L.bx * lat->L.by;
+            const bool one_kernel = lat->fused_launch && tt <= SPECIAL_BLOCK && SPECIAL_BLOCK % tt == 0;
+            // with timing on, the stream-collide kernel's dispatch records its
+            // own start and end (no event packets between the kernels)
+            hipEvent_t e0 = lat->timing ? lat->ev[2 * k] : nullptr, e1 = lat->timing ? lat->ev[2 * k + 1] : nullptr;
             // a spinning part is re-voxelized for this step's rotation first.
             // Its triangle hits come from stream3 when the previous step
             // launched them there; after the voxelization, stream3 clears
@@ -769,7 +780,15 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                 } else {
                     hip_check(launch_scatter(lat->L, a, st), "re-voxelization kernels");
                 }
+                // the rotation of the previous voxelization lets it leave the
+                // records of cells that stay far from the part as they are
+                if (lat->vox_prev_valid) {
+                    std::memcpy(a.vq, lat->vox_prev, sizeof a.vq);
+                    a.vq_valid = 1;
+                }
                 hip_check(launch_voxelize(lat->L, par, a, st), "re-voxelization kernels");
+                std::memcpy(lat->vox_prev, a.r, sizeof lat->vox_prev);
+                lat->vox_prev_valid = true;
                 if (lat->spin_overlap && done + k + 1 < nsteps) {
                     StepArgs an;
                     pack_step(still ? idn : steps[done + k + 1], step + 1, (uint32_t)(k + 1), an);
@@ -795,11 +814,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             } else if (!lat->fused_faces) {
                 hip_check(launch_faces(lat->L, par, a.seed, st), "face kernel");
             }
-            const uint32_t tt = lat->L.bx * lat->L.by;
-            // with timing on, the stream-collide kernel's dispatch records its
-            // own start and end (no event packets between the kernels)
-            hipEvent_t e0 = lat->timing ? lat->ev[2 * k] : nullptr, e1 = lat->timing ? lat->ev[2 * k + 1] : nullptr;
-            if (lat->fused_launch && tt <= SPECIAL_BLOCK && SPECIAL_BLOCK % tt == 0) {
+            if (one_kernel) {
                 // special and bulk blocks in one launch: no cross-stream wait
                 hip_check(launch_step_fused(lat->L, par, a, st, e0, e1), "stream-collide kernel");
             } else if (lat->concurrent) {
@@ -1244,8 +1259,12 @@ struct TuneGuard {
     uint64_t steps0;
     bool timing0, fused0;
     uint32_t bx0, by0;
+    bool vox_valid0;
+    float vox0[3][4];
     explicit TuneGuard(simlab_lattice *l)
-        : lat(l), steps0(l->steps_done), timing0(l->timing), fused0(l->fused_launch), bx0(l->L.bx), by0(l->L.by) {
+        : lat(l), steps0(l->steps_done), timing0(l->timing), fused0(l->fused_launch), bx0(l->L.bx), by0(l->L.by),
+          vox_valid0(l->vox_prev_valid) {
+        std::memcpy(vox0, l->vox_prev, sizeof vox0);
         bufs = {&l->pops, &l->flags, &l->rmask, &l->rq, &l->rfpost, &l->rmac, &l->rft, &l->vacc, &l->sacc, &l->qscr};
         uint64_t mem = 0;
         for (DevBuf *b : bufs) {
@@ -1267,6 +1286,8 @@ struct TuneGuard {
                 hip_check(hipMemcpyAsync(bufs[k]->p, saved[k]->p, bufs[k]->bytes, hipMemcpyDeviceToDevice, lat->stream),
                           "copy");
         lat->steps_done = steps0;
+        lat->vox_prev_valid = vox_valid0;
+        std::memcpy(lat->vox_prev, vox0, sizeof vox0);
     }
     void keep_tile(uint32_t bx, uint32_t by, bool fused) {
         bx0 = bx;
@@ -1431,6 +1452,7 @@ int simlab_cfd_lattice_restore(simlab_lattice *lat, const void *in, size_t bytes
             hip_check(hipStreamSynchronize(lat->stream), "GPU wait failed");
             lat->steps_done = h.steps_done;
             lat->adj_ready = false;
+            lat->vox_prev_valid = false;   // the next voxelization writes every box cell
             return SIMLAB_OK;
         } catch (const HipError &e) {
             return fail(e.code, e.what());
@@ -1521,13 +1543,17 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
         try {
             hip_check(hipSetDevice(b->device), "hipSetDevice");
             hipStream_t st = b->stream;
-            // Lattices above BATCH_SOLO_CELLS step with their own launches
-            // (at their own tile and register count; launch overhead is
-            // small beside their step), the others in the batched launch.
+            // One kernel per step for every member (the default), members
+            // ordered largest first so the small ones' tiles fill the large
+            // ones' tail; or (SIMLAB_CFD_BATCH=split) a special and a bulk
+            // kernel on two streams, members above BATCH_SOLO_CELLS stepping
+            // with their own launches.
+            const char *mode = std::getenv("SIMLAB_CFD_BATCH");
+            const bool fused = !(mode && std::string(mode) == "split");
             std::vector<size_t> mi;
             for (size_t i = 0; i < n; ++i) {
                 simlab_lattice *l = b->lats[i];
-                if (l->ncell <= BATCH_SOLO_CELLS) {
+                if (fused || l->ncell <= BATCH_SOLO_CELLS) {
                     mi.push_back(i);
                     continue;
                 }
@@ -1535,6 +1561,9 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                 if (rc != SIMLAB_OK) return rc;
             }
             if (mi.empty()) return SIMLAB_OK;
+            if (fused)
+                std::stable_sort(mi.begin(), mi.end(),
+                                 [&](size_t x, size_t y) { return b->lats[x]->ncell > b->lats[y]->ncell; });
             std::vector<simlab_lattice *> ml;
             for (size_t i : mi) ml.push_back(b->lats[i]);
             const size_t n = ml.size();   // the batched lattices from here on
@@ -1542,10 +1571,27 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
             std::vector<DevLattice> hl(n);
             std::vector<uint32_t> boff(2 * (n + 1), 0);
             const uint32_t threads = SPECIAL_BLOCK;
+            // in the one-kernel batch a member that steps with two kernels on
+            // its own (by default those of more than FUSED_MAX_CELLS cells)
+            // keeps its bulk kernel,
+            // at its own tile, on the second stream beside the batched kernel
+            // (which holds its special blocks); the others keep their x-face
+            // writes deferred to the small kernel after the step
+            std::vector<char> own_bulk(n, 0);
             for (size_t i = 0; i < n; ++i) {
                 hip_check(hipStreamSynchronize(ml[i]->stream), "GPU wait failed");
                 hl[i] = ml[i]->L;
-                hl[i].xdefer = 0;   // the batch kernels write every face cell themselves
+                {
+                    const uint32_t t0 = hl[i].bx * hl[i].by;
+                    own_bulk[i] = fused && !(ml[i]->fused_launch && t0 <= threads && threads % t0 == 0);
+                }
+                // the two-kernel batch writes every face cell in the step
+                if (!fused) hl[i].xdefer = 0;
+                if (own_bulk[i]) {
+                    boff[i + 1] = boff[i];
+                    boff[n + 1 + i + 1] = boff[n + 1 + i] + hl[i].wall_blocks + hl[i].hw_blocks;
+                    continue;
+                }
                 // the batch's blocks have SPECIAL_BLOCK threads: a tile that
                 // does not divide that is replaced (in the batch's copy):
                 // 128 x 1 for 16-bit storage, else whole rows of up to 256
@@ -1564,15 +1610,30 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                 boff[i + 1] = boff[i] + (uint32_t)((tiles + threads / tt - 1) / (threads / tt));
                 boff[n + 1 + i + 1] = boff[n + 1 + i] + hl[i].wall_blocks + hl[i].hw_blocks;
             }
+            if (fused) {
+                // the one-kernel batch reads [special offsets][bulk offsets]
+                std::vector<uint32_t> f(boff.size());
+                std::copy(boff.begin() + (n + 1), boff.end(), f.begin());
+                std::copy(boff.begin(), boff.begin() + (n + 1), f.begin() + (n + 1));
+                boff.swap(f);
+            }
             hip_check(hipMemcpyAsync(b->dlats.p, hl.data(), n * sizeof(DevLattice), hipMemcpyHostToDevice, st), "upload");
             hip_check(hipMemcpyAsync(b->boffs.p, boff.data(), boff.size() * 4, hipMemcpyHostToDevice, st), "upload");
             BatchLaunch bl;
             bl.lats = b->dlats.as<DevLattice>();
-            bl.boff_bulk = b->boffs.as<uint32_t>();
-            bl.boff_sp = b->boffs.as<uint32_t>() + (n + 1);
+            bl.fused = fused;
+            if (fused) {
+                bl.boff_sp = b->boffs.as<uint32_t>();
+                bl.boff_bulk = b->boffs.as<uint32_t>() + (n + 1);
+                bl.nsp = boff[n];
+                bl.nbulk = boff[2 * n + 1];
+            } else {
+                bl.boff_bulk = b->boffs.as<uint32_t>();
+                bl.boff_sp = b->boffs.as<uint32_t>() + (n + 1);
+                bl.nbulk = boff[n];
+                bl.nsp = boff[n + 1 + n];
+            }
             bl.nl = (uint32_t)n;
-            bl.nbulk = boff[n];
-            bl.nsp = boff[n + 1 + n];
             bl.threads = threads;
             bl.storage = hl[0].storage;
             bl.force = hl[0].forcing != 0;
@@ -1595,13 +1656,35 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                             hip_check(launch_faces(hl[i], par, a.seed, st), "face kernel");
                             hip_check(launch_faces(hl[i], 1 - par, (uint32_t)(step + 1) * 2654435761u, st), "face kernel");
                         }
-                    // special kernel first on the main stream, bulk kernel on
-                    // the side stream beside it (as a lattice's own split step)
-                    hip_check(hipEventRecord(b->fork, st), "event");
-                    hip_check(hipStreamWaitEvent(b->stream2, b->fork, 0), "event wait");
-                    hip_check(launch_batch_step(bl, par, a, st, b->stream2), "batched stream-collide kernels");
-                    hip_check(hipEventRecord(b->join, b->stream2), "event");
-                    hip_check(hipStreamWaitEvent(st, b->join, 0), "event wait");
+                    if (fused) {
+                        bool forked = false;
+                        for (size_t i = 0; i < n; ++i) {
+                            if (!own_bulk[i]) continue;
+                            if (!forked) {
+                                hip_check(hipEventRecord(b->fork, st), "event");
+                                hip_check(hipStreamWaitEvent(b->stream2, b->fork, 0), "event wait");
+                                forked = true;
+                            }
+                            hip_check(launch_bulk_planes(hl[i], par, a, b->stream2, 0, hl[i].nz, hl[i].nz, 0),
+                                      "stream-collide kernels");
+                        }
+                        hip_check(launch_batch_step(bl, par, a, st, st), "batched stream-collide kernel");
+                        if (forked) {
+                            hip_check(hipEventRecord(b->join, b->stream2), "event");
+                            hip_check(hipStreamWaitEvent(st, b->join, 0), "event wait");
+                        }
+                        for (size_t i = 0; i < n; ++i)
+                            if (hl[i].xdefer)
+                                hip_check(launch_xfaces(hl[i], par, a.seed + 2654435761u, st), "x-face kernel");
+                    } else {
+                        // special kernel first on the main stream, bulk kernel on
+                        // the side stream beside it (as a lattice's own split step)
+                        hip_check(hipEventRecord(b->fork, st), "event");
+                        hip_check(hipStreamWaitEvent(b->stream2, b->fork, 0), "event wait");
+                        hip_check(launch_batch_step(bl, par, a, st, b->stream2), "batched stream-collide kernels");
+                        hip_check(hipEventRecord(b->join, b->stream2), "event");
+                        hip_check(hipStreamWaitEvent(st, b->join, 0), "event wait");
+                    }
                 }
                 for (size_t i = 0; i < n; ++i) {
                     simlab_lattice *l = ml[i];

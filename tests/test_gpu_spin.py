@@ -99,6 +99,24 @@ def test_spinning_prop_matches_oracle(storage):
     assert du < (1e-12 if storage == sa.Storage.FP64 else 1e-5)
 
 
+def test_spinning_prop_two_kernel_step_matches_oracle(monkeypatch):
+    # the two-kernel step (special and bulk kernels on two streams) of the
+    # lattices too large for the one-kernel step, on a spinning part:
+    # records bit-identical to the oracle, forces to the FP32 tolerance
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", "split")
+    setup, geom, meta = small_prop(sa.Storage.FP32)
+    say("case spin two-kernel step", setup.dims, "box", meta["box_lo"], meta["box_n"])
+    lat = sa.Lattice(setup, geom)
+    ora = O.OracleLattice(setup, geom, store=O.F32)
+    drv = [W.SpinDriver((0.0, 0.0, 1.0), meta["omega"], ramp_revs=0.0) for _ in range(2)]
+    for n in (1, 12, 27):
+        hg, ho = lat.run(drv[0].next(n)), ora.run(drv[1].next(n))
+        assert np.array_equal(lat.records()[:, :11], ora.records()[:, :11])
+        df = np.max(np.abs(hg - ho) / (np.max(np.abs(ho), axis=0) + 1e-30))
+        say(f"  {n} steps: force/torque max rel diff to the oracle {df:.2e}")
+        assert df < 2e-3
+
+
 def test_spun_up_prop_feels_torque_against_its_rotation():
     setup, geom, meta = small_prop(sa.Storage.FP32)
     setup.u_inf = (0.0, 0.0, 0.0)
