@@ -48,6 +48,13 @@ struct DevLattice {
 
     void *pops;
     uint32_t *flags;             // bits 0-1 type (0 fluid, 1 solid, 2 velocity face, 3 open face), bits 2.. record+1
+    // bit g set: the POP_GROUP cells of group g (cells 64 g .. 64 g + 63)
+    // are plain fluid cells (flags 0) whose flags never change, so a wave of
+    // them needs no flag load (null: not used)
+    const uint32_t *plain;
+    // 1: bulk blocks are taken in XCD order (the GPU deals workgroups to its
+    // 8 XCDs in turn; the swizzle gives each XCD a contiguous run of tiles)
+    uint32_t xcd_swizzle;
 
     uint32_t nrec;
     uint32_t *rmask;             // [nrec]

@@ -1070,6 +1070,16 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_special(DevLattice L, StepArg
 }
 
 // ---------------------------------------------------------------- bulk
+// Block b of nb taken in XCD order: the GPU deals consecutive workgroups to
+// its 8 XCDs in turn (b % 8); this gives XCD k the k-th contiguous run of
+// the nb tiles, so neighbouring tiles (which share the cache lines that
+// straddle their x-edges in a parity-1 step) meet in one L2. A bijection on
+// [0, nb).
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t nb) {
+    const uint32_t xcd = b & 7u, j = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q) + j;
+}
+
 // Every fluid cell with no wall record (face sources and halfway wall cells
 // included). Grid:
 // (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
@@ -1080,13 +1090,29 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_special(DevLattice L, StepArg
 template <class C, int P, bool FORCE, bool SPONGE>
 __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &a, uint32_t bxy, uint32_t z, uint32_t tid) {
     using T = typename C::T;
+    // A wave is 64 consecutive x of one row (bx is a multiple of 64, tiles
+    // are whole waves): the tile, the row and the plane are the same for all
+    // its lanes, so they (and the row and plane neighbour offsets) live in
+    // scalar registers; only x differs per lane.
+    bxy = __builtin_amdgcn_readfirstlane(bxy);
+    z = __builtin_amdgcn_readfirstlane(z);
     const uint32_t xb = bxy % L.xblocks, yb = bxy / L.xblocks;
-    const uint32_t tx = tid % L.bx, ty = tid / L.bx;
+    const uint32_t tx = tid % L.bx, ty = __builtin_amdgcn_readfirstlane(tid / L.bx);
     const uint32_t x = xb * L.bx + tx;
     const uint32_t y = yb * L.by + ty;
     const bool inside = x < L.nx && y < L.ny && ty < L.by;
     const uint32_t n = (z * L.ny + y) * L.nx + x;
-    const uint32_t fg = inside ? L.flags[n] : 1u;
+    // a wave is 64 consecutive cells of one row (bx is a multiple of 64):
+    // where they all lie in plain-fluid groups the flags are known to be 0
+    uint32_t fg;
+    if (L.plain && __all(inside)) {
+        const uint32_t n0 = __builtin_amdgcn_readfirstlane(n);
+        const uint32_t g0 = n0 / POP_GROUP, g1 = (n0 + 63u) / POP_GROUP;
+        const bool pl = ((L.plain[g0 >> 5] >> (g0 & 31u)) & 1u) && ((L.plain[g1 >> 5] >> (g1 & 31u)) & 1u);
+        fg = pl ? 0u : L.flags[n];
+    } else {
+        fg = inside ? L.flags[n] : 1u;
+    }
     // The populations are loaded without waiting for the flag, so the two
     // round trips overlap: every cell that may collide (all but the face
     // cells, known from the coordinates) loads, every index being inside the
@@ -1175,9 +1201,15 @@ struct ZMap {
 template <class C, int P, bool FORCE, bool SPONGE>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(bulk_waves<C, P, FORCE, SPONGE>())))
 k_bulk(DevLattice L, StepArgs a, ZMap zm) {
-    uint32_t z = zm.zbase + blockIdx.y;
+    uint32_t bxy = blockIdx.x, zl = blockIdx.y;
+    if (L.xcd_swizzle) {
+        const uint32_t lin = xcd_order(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+        bxy = lin % gridDim.x;
+        zl = lin / gridDim.x;
+    }
+    uint32_t z = zm.zbase + zl;
     if (z >= zm.zskip_at) z += zm.zskip_len;
-    bulk_block<C, P, FORCE, SPONGE>(L, a, blockIdx.x, z, threadIdx.y * blockDim.x + threadIdx.x);
+    bulk_block<C, P, FORCE, SPONGE>(L, a, bxy, z, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
 // One launch per step, one stream: blocks [0, wall_blocks + hw_blocks) do
@@ -1202,7 +1234,9 @@ k_step(DevLattice L, StepArgs a) {
         return;
     }
     const uint32_t tt = L.bx * L.by;   // divides SPECIAL_BLOCK (checked by the host)
-    const uint32_t g = (blockIdx.x - nsp) * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
+    uint32_t b = blockIdx.x - nsp;
+    if (L.xcd_swizzle) b = xcd_order(b, gridDim.x - nsp);
+    const uint32_t g = b * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;
     if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
     bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
@@ -1265,7 +1299,10 @@ k_step_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ b
     const uint32_t b = blockIdx.x - nsp;
     const uint32_t i = batch_member(bulk, nl, b);
     const DevLattice &L = lats[i];
-    const uint32_t local = __builtin_amdgcn_readfirstlane(b - bulk[i]);
+    uint32_t local = __builtin_amdgcn_readfirstlane(b - bulk[i]);
+    // XCD order within the member's blocks (its first block's XCD only
+    // relabels the runs)
+    if (L.xcd_swizzle) local = xcd_order(local, bulk[i + 1] - bulk[i]);
     const uint32_t tt = L.bx * L.by;
     const uint32_t g = local * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;

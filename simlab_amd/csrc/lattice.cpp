@@ -129,6 +129,7 @@ struct simlab_lattice {
     uint64_t nbulk = 0;              // cells the bulk kernel updates
     DevLattice L{};
     DevBuf xsrc;                     // deferred x-face sources (L.xsrc)
+    DevBuf plain;                    // plain-fluid groups (L.plain)
     DevBuf pops, flags, rmask, rcell, rq, rfpost, rmac, rft, faces, hist, hw_cells, partials,
         counter, dump, mass_part, mass_out, occ, tris, qscr, near, verts, vacc, slicedef, sacc;
     bool fused_faces = true;   // face update fused into the previous step
@@ -571,6 +572,34 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         if (bytes) hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), "upload");
     };
     up(lat->flags, h_flags.data(), n * 4);
+    {
+        // groups of POP_GROUP cells that are all plain fluid (flags 0) and
+        // stay so: the swept box of a spinning part is re-voxelized every
+        // step, so no group meeting it counts
+        const uint64_t ng = (n + POP_GROUP - 1) / POP_GROUP;
+        std::vector<uint32_t> pl((ng + 31) / 32, 0u);
+        for (uint64_t g = 0; g < ng; ++g) {
+            bool ok = true;
+            const uint64_t c1 = std::min<uint64_t>(n, (g + 1) * POP_GROUP);
+            for (uint64_t c = g * POP_GROUP; c < c1 && ok; ++c) ok = h_flags[c] == 0u;
+            if (ok) pl[g >> 5] |= 1u << (g & 31);
+        }
+        if (geom_kind == SIMLAB_GEOM_SPIN) {
+            const uint64_t nx = su->dims[0], ny = su->dims[1];
+            for (uint64_t z = g->box_lo[2]; z < (uint64_t)g->box_lo[2] + g->box_n[2]; ++z)
+                for (uint64_t y = g->box_lo[1]; y < (uint64_t)g->box_lo[1] + g->box_n[1]; ++y) {
+                    const uint64_t r0 = (z * ny + y) * nx + g->box_lo[0];
+                    for (uint64_t gg = r0 / POP_GROUP; gg <= (r0 + g->box_n[0] - 1) / POP_GROUP; ++gg)
+                        pl[gg >> 5] &= ~(1u << (gg & 31));
+                }
+        }
+        lat->plain.alloc(pl.size() * 4, mem);
+        up(lat->plain, pl.data(), pl.size() * 4);
+        const char *e = std::getenv("SIMLAB_CFD_PLAIN");
+        lat->L.plain = (e && e[0] == '0') ? nullptr : lat->plain.as<uint32_t>();
+        const char *sw = std::getenv("SIMLAB_CFD_SWIZZLE");
+        lat->L.xcd_swizzle = (sw && sw[0] == '0') ? 0u : 1u;
+    }
     up(lat->rmask, h_mask.data(), (size_t)nrec * 4);
     up(lat->rcell, h_cell.data(), (size_t)nrec * 4);
     up(lat->rq, h_q.data(), h_q.size() * 4);

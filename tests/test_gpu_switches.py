@@ -45,3 +45,20 @@ def test_split_and_one_kernel_steps_agree(monkeypatch, storage):
     else:
         d = np.abs(pf.view(np.float16).astype(np.float64) - ps.view(np.float16).astype(np.float64)).max() / 32768.0
         assert d < 1e-4
+
+
+@pytest.mark.parametrize("launch", ["fused", "split"])
+def test_plain_groups_and_xcd_order_change_nothing(monkeypatch, launch):
+    # skipping the flag loads of plain-fluid groups and taking the tiles in
+    # XCD order change which loads run and in which order blocks start, not
+    # a single arithmetic operation: bit-identical populations and forces
+    print(f"case plain/swizzle on/off, {launch} launch", flush=True)
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", launch)
+    out = {}
+    for plain, swz in (("1", "1"), ("0", "0")):
+        monkeypatch.setenv("SIMLAB_CFD_PLAIN", plain)
+        monkeypatch.setenv("SIMLAB_CFD_SWIZZLE", swz)
+        setup, geom, meta = W.sphere_channel((128, 64, 48), 6.0, 30.0, 200.0, 0.05, 0.1, sa.Storage.FP16, subdiv=3)
+        lat = sa.Lattice(setup, geom)
+        out[plain] = (lat.run_still(25), lat.get_pops())
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
