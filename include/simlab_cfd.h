@@ -324,6 +324,18 @@ int simlab_cfd_voxelize_static(const double *verts, uint64_t nverts,
                                const uint32_t *tris, uint64_t ntris,
                                const uint64_t dims[3],
                                simlab_static_voxels **out);
+/* `voxelize_static` with its two expensive parts on the GPU (the current
+ * device): the ray-parity inside test (one thread per scan line) and the
+ * wall-link search (one thread per examined point). The same double
+ * precision operations in the same order as the CPU version, no fused
+ * multiply-adds: the output is identical, solid cells, links and q bits.
+ * New: the reference voxelizes on the CPU only. A scan line crossing more
+ * than 64 surfaces makes the inside test fall back to the CPU for that
+ * call. */
+int simlab_cfd_voxelize_static_gpu(const double *verts, uint64_t nverts,
+                                   const uint32_t *tris, uint64_t ntris,
+                                   const uint64_t dims[3],
+                                   simlab_static_voxels **out);
 /* Views into a voxelization (valid until it is freed); lo/hi: examined box. */
 int simlab_cfd_voxels_view(const simlab_static_voxels *v, uint64_t *nsolid,
                            const uint32_t **solid, uint64_t *nwall,

@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH):
         "simlab_cfd_adjoint_forward": (I, [V, C.c_size_t, C.c_size_t, C.c_size_t, I, C.POINTER(D), P]),
         "simlab_cfd_adjoint_backward": (I, [V, P]),
         "simlab_cfd_voxelize_static": (I, [P, U64, P, U64, C.POINTER(U64), C.POINTER(V)]),
+        "simlab_cfd_voxelize_static_gpu": (I, [P, U64, P, U64, C.POINTER(U64), C.POINTER(V)]),
         "simlab_cfd_voxelize_spin": (I, [P, U64, P, U64, C.POINTER(D), D, C.POINTER(V)]),
         "simlab_cfd_spin_voxels_view": (I, [V, C.POINTER(D), C.POINTER(D), C.POINTER(U64), C.POINTER(U64), C.POINTER(P),
                                             C.POINTER(U64), C.POINTER(P), C.POINTER(D), C.POINTER(D)]),
@@ -328,13 +329,16 @@ def _surface_arrays(verts, tris):
     return v, t
 
 
-def voxelize_static(verts, tris, dims) -> StaticVoxels:
-    """voxel.rs:348-352: solid cells and wall links of a surface in lattice coordinates."""
+def voxelize_static(verts, tris, dims, gpu: bool = False) -> StaticVoxels:
+    """voxel.rs:348-352: solid cells and wall links of a surface in lattice
+    coordinates; gpu: the inside test and the link search on the GPU
+    (simlab_cfd_voxelize_static_gpu, the same output)."""
     lib = load_library()
     v, t = _surface_arrays(verts, tris)
     d = (C.c_uint64 * 3)(*[int(x) for x in dims])
     h = C.c_void_p()
-    _check(lib, lib.simlab_cfd_voxelize_static(_ptr(v), len(v), _ptr(t), len(t), d, C.byref(h)))
+    fn = lib.simlab_cfd_voxelize_static_gpu if gpu else lib.simlab_cfd_voxelize_static
+    _check(lib, fn(_ptr(v), len(v), _ptr(t), len(t), d, C.byref(h)))
     try:
         ns, nw = C.c_uint64(), C.c_uint64()
         ps, pw = C.c_void_p(), C.c_void_p()

@@ -48,6 +48,9 @@ bool total_cmp_less(double x, double y);
 bool seg_tri(const double o[3], const double d[3], const double *t0, const double *t1, const double *t2, double &tt);
 std::vector<uint8_t> inside_grid_public(const Surface &s, const double origin[3], double h, const size_t dims[3]);
 StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]);
+// the same voxelization with the inside test and the link search on the GPU
+// (the current device); identical output
+StaticVoxels voxelize_static_gpu(const Surface &s, const size_t dims[3]);
 Surface subdivide(const Surface &s, double max_edge);
 SpinVoxels voxelize_spin(const Surface &s, const double axis[3], double h);
 

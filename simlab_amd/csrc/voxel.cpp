@@ -13,11 +13,16 @@
 //    crossing a sheet thinner than a cell are walls from both sides.
 
 #include "simlab_internal.h"
+#include "voxel_gpu.h"
+
+#include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
+#include <string>
 #include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
@@ -76,13 +81,19 @@ void mark_line(const Grid &g, int a, int b, int c, size_t j, size_t k, const Hit
     }
 }
 
-// voxel.rs:97-184
-void parity_axis(const Surface &s, const Grid &g, int a, std::atomic<uint32_t> *votes) {
-    int b = (a + 1) % 3, c = (a + 2) % 3;
-    size_t nb = g.dims[b], nc = g.dims[c];
+// vertices in grid units (voxel.rs:103-106)
+std::vector<P3> grid_verts(const Surface &s, const Grid &g) {
     std::vector<P3> gv(s.verts.size());
     for (size_t v = 0; v < s.verts.size(); ++v)
         for (int k = 0; k < 3; ++k) gv[v][k] = (s.verts[v][k] - g.origin[k]) / g.h;
+    return gv;
+}
+
+// the triangles whose c-range meets row k of the scan lines along axis a
+// (voxel.rs:107-120), in triangle order
+std::vector<std::vector<uint32_t>> axis_rows(const Surface &s, const Grid &g, const std::vector<P3> &gv, int a) {
+    int c = (a + 2) % 3;
+    size_t nc = g.dims[c];
     std::vector<std::vector<uint32_t>> rows(nc);
     for (size_t t = 0; t < s.tris.size(); ++t) {
         double lo = INFINITY, hi = -INFINITY;
@@ -97,6 +108,15 @@ void parity_axis(const Surface &s, const Grid &g, int a, std::atomic<uint32_t> *
         size_t k1 = std::min((size_t)k1f, nc ? nc - 1 : 0);
         for (size_t k = k0; k <= k1 && k < nc; ++k) rows[k].push_back((uint32_t)t);
     }
+    return rows;
+}
+
+// voxel.rs:97-184
+void parity_axis(const Surface &s, const Grid &g, int a, std::atomic<uint32_t> *votes) {
+    int b = (a + 1) % 3, c = (a + 2) % 3;
+    size_t nb = g.dims[b], nc = g.dims[c];
+    std::vector<P3> gv = grid_verts(s, g);
+    std::vector<std::vector<uint32_t>> rows = axis_rows(s, g, gv, a);
 #pragma omp parallel
     {
         std::vector<Hit> hits;
@@ -284,8 +304,90 @@ std::vector<uint8_t> inside_grid_public(const Surface &s, const double origin[3]
     return inside_grid(s, g);
 }
 
-// voxel.rs:348-352 + 363-490 (D3Q19 links)
-StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
+namespace {
+struct Thin {
+    size_t nl;
+    int d;
+    float q;
+};
+struct Cell {
+    size_t l;
+    uint32_t mask;
+    float q[27];
+    std::vector<Thin> thin;
+};
+
+void hcheck(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("GPU voxelizer: ") + what + ": " + hipGetErrorString(e));
+}
+// device memory owned by the GPU voxelizer for one call
+struct DMem {
+    void *p = nullptr;
+    DMem() = default;
+    DMem(const DMem &) = delete;
+    DMem &operator=(const DMem &) = delete;
+    ~DMem() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T *alloc(size_t n) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        hcheck(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 16)), "hipMalloc");
+        return (T *)p;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+template <class T> void upload(DMem &d, const std::vector<T> &v, hipStream_t st) {
+    d.alloc<T>(v.size());
+    if (!v.empty()) hcheck(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+}
+
+// the inside test on the GPU: the same votes as inside_grid. False if a
+// scan line crossed more surfaces than a thread keeps (the caller then
+// takes the CPU test); d_inside keeps the result on the device.
+bool inside_grid_gpu(const Surface &s, const Grid &g, const DMem &d_tris, DMem &d_inside, std::vector<uint8_t> &inside,
+                     hipStream_t st) {
+    const size_t n = g.len();
+    DMem d_votes, d_over, d_gv, d_rs, d_rl;
+    uint32_t *votes = d_votes.alloc<uint32_t>(n);
+    uint32_t *over = d_over.alloc<uint32_t>(1);
+    hcheck(hipMemsetAsync(votes, 0, n * 4, st), "clear");
+    hcheck(hipMemsetAsync(over, 0, 4, st), "clear");
+    std::vector<P3> gv = grid_verts(s, g);
+    std::vector<double> gvf(gv.size() * 3);
+    for (size_t v = 0; v < gv.size(); ++v)
+        for (int k = 0; k < 3; ++k) gvf[3 * v + k] = gv[v][k];
+    upload(d_gv, gvf, st);
+    const uint64_t gdims[3] = {g.dims[0], g.dims[1], g.dims[2]};
+    for (int a = 0; a < 3; ++a) {
+        std::vector<std::vector<uint32_t>> rows = axis_rows(s, g, gv, a);
+        std::vector<uint32_t> start(rows.size() + 1, 0), list;
+        for (size_t k = 0; k < rows.size(); ++k) {
+            start[k + 1] = start[k] + (uint32_t)rows[k].size();
+            list.insert(list.end(), rows[k].begin(), rows[k].end());
+        }
+        upload(d_rs, start, st);
+        upload(d_rl, list, st);
+        hcheck(gvox_parity(d_gv.as<double>(), d_tris.as<uint32_t>(), d_rs.as<uint32_t>(), d_rl.as<uint32_t>(), a, gdims,
+                           votes, over, st),
+               "parity kernel");
+        hcheck(hipStreamSynchronize(st), "parity kernel");   // the row lists are freed with the next upload
+    }
+    uint32_t ov = 0;
+    hcheck(hipMemcpy(&ov, over, 4, hipMemcpyDeviceToHost), "read");
+    if (ov) return false;
+    uint8_t *ins = d_inside.alloc<uint8_t>(n);
+    hcheck(gvox_majority(votes, n, ins, st), "majority kernel");
+    inside.resize(n);
+    hcheck(hipMemcpyAsync(inside.data(), ins, n, hipMemcpyDeviceToHost, st), "read");
+    hcheck(hipStreamSynchronize(st), "majority kernel");
+    return true;
+}
+}  // namespace
+
+// voxel.rs:348-352 + 363-490 (D3Q19 links). gpu: the inside test and the
+// link search on the GPU (the same decisions; voxel_gpu.hip)
+StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool gpu) {
     StaticVoxels out;
     const int nd = 19, rest = 0;
     const double maxlen = std::sqrt(2.0);
@@ -310,7 +412,22 @@ StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
     size_t gd[3];
     for (int k = 0; k < 3; ++k) gd[k] = (size_t)(shi[k] - slo[k] + 1);
     Grid g{{(double)slo[0], (double)slo[1], (double)slo[2]}, 1.0, {gd[0], gd[1], gd[2]}};
-    std::vector<uint8_t> inside = inside_grid(s, g);
+    std::vector<uint8_t> inside;
+    hipStream_t st = nullptr;
+    DMem d_tris, d_inside, d_verts;
+    bool on_gpu = false;
+    if (gpu) {
+        std::vector<uint32_t> tf(s.tris.size() * 3);
+        std::vector<double> vf(s.verts.size() * 3);
+        for (size_t t = 0; t < s.tris.size(); ++t)
+            for (int k = 0; k < 3; ++k) tf[3 * t + k] = s.tris[t][k];
+        for (size_t v = 0; v < s.verts.size(); ++v)
+            for (int k = 0; k < 3; ++k) vf[3 * v + k] = s.verts[v][k];
+        upload(d_tris, tf, st);
+        upload(d_verts, vf, st);
+        on_gpu = inside_grid_gpu(s, g, d_tris, d_inside, inside, st);
+    }
+    if (!on_gpu) inside = inside_grid(s, g);
     Bins bins;
     bins.build(s, {(double)slo[0] - 0.5, (double)slo[1] - 0.5, (double)slo[2] - 0.5}, gd, maxlen + 0.01);
     auto in_domain = [&](size_t i, size_t j, size_t k) {
@@ -339,25 +456,64 @@ StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
         if (i < 0 || j < 0 || k < 0 || i >= (int64_t)gd[0] || j >= (int64_t)gd[1] || k >= (int64_t)gd[2]) return false;
         return inside[g.idx((size_t)i, (size_t)j, (size_t)k)] != 0;
     };
-    struct Thin {
-        size_t nl;
-        int d;
-        float q;
-    };
-    struct Cell {
-        size_t l;
-        uint32_t mask;
-        float q[27];
-        std::vector<Thin> thin;
-    };
+    std::vector<Cell> cells;
+    if (on_gpu) {
+        // every fluid point with a wall link, in any order; sorted by point below
+        DMem d_bs, d_bl, d_out, d_cnt;
+        upload(d_bs, bins.start, st);
+        upload(d_bl, bins.list, st);
+        GvoxLinkArgs a{};
+        a.verts = d_verts.as<double>();
+        a.tris = d_tris.as<uint32_t>();
+        a.inside = d_inside.as<uint8_t>();
+        a.bin_start = d_bs.as<uint32_t>();
+        a.bin_list = d_bl.as<uint32_t>();
+        for (int k = 0; k < 3; ++k) {
+            a.bin_lo[k] = bins.lo[k];
+            a.bin_dims[k] = bins.dims[k];
+            a.gd[k] = gd[k];
+            a.slo[k] = slo[k];
+            a.dims[k] = dims[k];
+        }
+        a.count = d_cnt.alloc<unsigned long long>(1);
+        a.capacity = std::min<uint64_t>(glen, (uint64_t)1 << 20);
+        unsigned long long found = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            a.out = d_out.alloc<GvoxCell>(a.capacity);
+            hcheck(hipMemsetAsync(a.count, 0, 8, st), "clear");
+            hcheck(gvox_links(a, st), "link kernel");
+            hcheck(hipMemcpyAsync(&found, a.count, 8, hipMemcpyDeviceToHost, st), "read");
+            hcheck(hipStreamSynchronize(st), "link kernel");
+            if (found <= a.capacity) break;
+            a.capacity = found;   // room for all of them: once more
+        }
+        std::vector<GvoxCell> gc(found);
+        if (found) hcheck(hipMemcpy(gc.data(), a.out, found * sizeof(GvoxCell), hipMemcpyDeviceToHost), "read");
+        std::sort(gc.begin(), gc.end(), [](const GvoxCell &x, const GvoxCell &y) { return x.l < y.l; });
+        cells.resize(found);
+        for (size_t r = 0; r < found; ++r) {
+            Cell &c = cells[r];
+            c.l = gc[r].l;
+            c.mask = gc[r].mask;
+            for (int d = 0; d < 27; ++d) c.q[d] = d < 19 ? gc[r].q[d] : 1.0f;
+            size_t i, j, k;
+            rel(c.l, i, j, k);
+            for (int d = 1; d < 19; ++d)
+                if (gc[r].thin_q[d] >= 0.0f) {
+                    const size_t ni = (size_t)((int64_t)i + CV[d][0]), nj = (size_t)((int64_t)j + CV[d][1]),
+                                 nk = (size_t)((int64_t)k + CV[d][2]);
+                    c.thin.push_back({g.idx(ni, nj, nk), opp(d), gc[r].thin_q[d]});
+                }
+        }
+    }
     int nthreads = 1;
 #ifdef _OPENMP
     nthreads = omp_get_max_threads();
 #endif
-    std::vector<std::vector<Cell>> parts(nthreads);
+    std::vector<std::vector<Cell>> parts(on_gpu ? 0 : nthreads);
     // contiguous static chunks per thread keep the concatenation in l order
-#pragma omp parallel num_threads(nthreads)
-    {
+#pragma omp parallel num_threads(nthreads) if (!on_gpu)
+    if (!on_gpu) {
         int tid = 0;
 #ifdef _OPENMP
         tid = omp_get_thread_num();
@@ -409,7 +565,6 @@ StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
             if (cell.mask) parts[tid].push_back(std::move(cell));
         }
     }
-    std::vector<Cell> cells;
     for (auto &pt : parts)
         for (auto &c : pt) cells.push_back(std::move(c));
     // make thin-sheet links reciprocal (voxel.rs:462-480)
@@ -454,6 +609,9 @@ StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
     }
     return out;
 }
+
+StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) { return voxelize_static_impl(s, dims, false); }
+StaticVoxels voxelize_static_gpu(const Surface &s, const size_t dims[3]) { return voxelize_static_impl(s, dims, true); }
 
 // voxel.rs:562-584: split triangles 1 -> 4 until no edge exceeds max_edge
 Surface subdivide(const Surface &s, double max_edge) {
@@ -562,6 +720,25 @@ int simlab_cfd_voxelize_static(const double *verts, uint64_t nverts, const uint3
         size_t d[3] = {(size_t)dims[0], (size_t)dims[1], (size_t)dims[2]};
         auto *v = new simlab_static_voxels;
         v->v = simlab::voxelize_static(s, d);
+        *out = v;
+        return SIMLAB_OK;
+    });
+}
+
+int simlab_cfd_voxelize_static_gpu(const double *verts, uint64_t nverts, const uint32_t *tris, uint64_t ntris,
+                                   const uint64_t dims[3], simlab_static_voxels **out) {
+    return simlab::guarded([&]() -> int {
+        if (!out || !dims || (nverts && !verts) || (ntris && !tris)) return simlab::fail(SIMLAB_ERR_INVALID, "null argument");
+        simlab::Surface s;
+        if (!make_surface(verts, nverts, tris, ntris, s)) return simlab::fail(SIMLAB_ERR_INVALID, "triangle index out of range");
+        size_t d[3] = {(size_t)dims[0], (size_t)dims[1], (size_t)dims[2]};
+        auto *v = new simlab_static_voxels;
+        try {
+            v->v = simlab::voxelize_static_gpu(s, d);
+        } catch (...) {
+            delete v;
+            throw;
+        }
         *out = v;
         return SIMLAB_OK;
     });
