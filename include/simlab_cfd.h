@@ -356,6 +356,99 @@ int simlab_cfd_spin_voxels_view(const simlab_spin_voxels *v, double occ_lo[3], d
                                 double *radius, double axial[2]);
 void simlab_cfd_spin_voxels_free(simlab_spin_voxels *v);
 
+/* `voxelize_static27` (voxel.rs:354-359, 363-490): the same voxelization
+ * with the D3Q27 links (direction i = (i % 3 - 1, (i / 3) % 3 - 1,
+ * i / 9 - 1), d3q27.rs:1-30), for the refined solver. Host CPU. */
+int simlab_cfd_voxelize_static27(const double *verts, uint64_t nverts,
+                                 const uint32_t *tris, uint64_t ntris,
+                                 const uint64_t dims[3],
+                                 simlab_static_voxels **out);
+
+/* ------------------------------------------------------------------------
+ * The refined D3Q27 cumulant solver (crates/cfd/src/mlattice.rs,
+ * mshader.rs, d3q27.rs): nested node-aligned boxes, each level half the
+ * cell size of its parent and stepped twice per parent step (the same
+ * lattice velocity on every level). The still part lives on the finest
+ * level, the domain faces on the coarsest. Lattice units of the finest
+ * level unless stated. Storage FP32, FP16 (deviations from the weights,
+ * scaled by 2^15, the reference's per-store stochastic rounding) or FP64
+ * (the oracle's arithmetic; new). */
+#define SIMLAB_MULTI_MAX_LEVELS 8
+
+/* `LevelBox`, mlattice.rs:15-21: lo is the parent node of this level's
+ * node 0, n the parent nodes it spans per axis (dims = 2 (n - 1) + 1). */
+typedef struct simlab_level_box {
+    int32_t lo[3];
+    uint32_t n[3];
+} simlab_level_box;
+
+/* `MultiSetup`, mlattice.rs:23-48; field meaning and units as there. */
+typedef struct simlab_multi_setup {
+    uint32_t dims0[3];           /* coarsest level size in its cells        */
+    uint32_t nboxes;             /* levels - 1                              */
+    simlab_level_box boxes[SIMLAB_MULTI_MAX_LEVELS - 1];
+    int32_t faces[6];            /* simlab_face_bc, -x +x -y +y -z +z       */
+    double nu0;                  /* viscosity on the coarsest level         */
+    double smagorinsky;
+    double u_inf[3];
+    double u_init[3];
+    double body_force0[3];       /* coarsest level; halves per level        */
+    uint32_t sponge[6];          /* coarsest level cells                    */
+    double sponge_tau;
+    double sponge_sigma;
+    int32_t absorb_velocity;
+    int32_t total_pressure_inlet;
+    int32_t storage;             /* FP32, FP16 or FP64                      */
+    int32_t debug_dump;
+    double center[3];            /* moment reference, finest coordinates    */
+} simlab_multi_setup;
+
+/* Public fields of `MultiLattice` (mlattice.rs:139-160) and its levels. */
+typedef struct simlab_multi_info {
+    uint32_t levels;
+    uint32_t sub_steps;          /* finest steps per coarsest step          */
+    uint64_t fine_steps;
+    uint64_t memory_bytes;
+    uint32_t nrec;
+    uint32_t nvert;
+    uint32_t npix;
+    int32_t storage;
+    uint32_t dims[SIMLAB_MULTI_MAX_LEVELS][3];
+    uint64_t cells[SIMLAB_MULTI_MAX_LEVELS];
+    double tau[SIMLAB_MULTI_MAX_LEVELS];   /* relaxation time per level    */
+} simlab_multi_info;
+
+typedef struct simlab_multi simlab_multi;   /* opaque */
+
+/* `MultiLattice::new` (mlattice.rs:185-690). geom: SIMLAB_GEOM_NONE or
+ * SIMLAB_GEOM_STATIC with D3Q27 wall cells in finest-level cells (from
+ * simlab_cfd_voxelize_static27); probes may be NULL (slice origins and
+ * steps in finest-level node coordinates). */
+int simlab_cfd_multi_new(const simlab_multi_setup *setup, const simlab_geometry *geom,
+                         const simlab_probes *probes, simlab_multi **out);
+void simlab_cfd_multi_free(simlab_multi *m);
+int simlab_cfd_multi_info(const simlab_multi *m, simlab_multi_info *out);
+/* `MultiLattice::run` (mlattice.rs:725-784): nsteps finest steps (a multiple
+ * of sub_steps); force and moment on the part after each finest step
+ * (nsteps x 6 doubles, finest lattice units), summed in FP64. */
+int simlab_cfd_multi_run(simlab_multi *m, const simlab_step_in *steps, size_t nsteps, double *ft_out);
+int simlab_cfd_multi_run_still(simlab_multi *m, size_t nsteps, double *ft_out);
+/* `macros(l)` (mlattice.rs:913-930): (rho, ux, uy, uz) of every cell of
+ * level l in that level's units, NaN where not a stepped fluid cell; needs
+ * debug_dump. out: cells(l) x 4 floats. */
+int simlab_cfd_multi_macros(simlab_multi *m, uint32_t level, float *out);
+/* `records` (mlattice.rs:932-934): max(nrec, 1) x 24 words (mask, cell,
+ * 13 words of f16 q pairs by moving-direction slot, F, T as f32 bits). */
+int simlab_cfd_multi_records(simlab_multi *m, uint32_t *out);
+/* `reset_samples`, `vertex_means`, `slice_means` (mlattice.rs:868-911). */
+int simlab_cfd_multi_reset_samples(simlab_multi *m);
+int simlab_cfd_multi_vertex_means(simlab_multi *m, float *out);
+int simlab_cfd_multi_slice_means(simlab_multi *m, float *out);
+/* The populations a level's next step reads (the source set), decoded to
+ * deviations f - w as doubles, direction i of cell n at i * cells + n. For
+ * tests. out: 27 x cells(l) doubles. */
+int simlab_cfd_multi_get_pops(simlab_multi *m, uint32_t level, double *out);
+
 /* `inside_grid` (voxel.rs:73-95): inside flags (0/1) for the grid points
  * origin + h * (i, j, k), i fastest. out: dims[0]*dims[1]*dims[2] bytes. */
 int simlab_cfd_inside_grid(const double *verts, uint64_t nverts,

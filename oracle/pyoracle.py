@@ -265,3 +265,156 @@ def refill_stress_f64(fpost18, mac4):
     out = np.zeros(6)
     lib().oracle_refill_stress_f64(_ptr(f), _ptr(m), len(f), _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------- refined D3Q27 solver
+_mlib_done = False
+
+
+def mlib():
+    """liboracle.so with the signatures of the D3Q27 restatement (mlbm_oracle.cpp)."""
+    global _mlib_done
+    L = lib()
+    if not _mlib_done:
+        from simlab_amd import mlattice as ml
+        V, P = C.c_void_p, C.c_void_p
+        L.moracle_new.restype = V
+        L.moracle_new.argtypes = [C.POINTER(ml._MultiSetup), C.POINTER(sa._Geometry), C.POINTER(sa._Probes), C.c_int,
+                                  C.c_int]
+        L.moracle_free.argtypes = [V]
+        L.moracle_last_error.restype = C.c_char_p
+        L.moracle_run.restype = C.c_int
+        L.moracle_run.argtypes = [V, C.POINTER(sa._StepIn), C.c_uint64, P]
+        L.moracle_run_still.restype = C.c_int
+        L.moracle_run_still.argtypes = [V, C.c_uint64, P]
+        for n in ("moracle_levels", "moracle_nrec", "moracle_npix"):
+            getattr(L, n).restype = C.c_uint32
+            getattr(L, n).argtypes = [V]
+        L.moracle_level_dims.argtypes = [V, P]
+        L.moracle_set_pops.argtypes = [V, C.c_uint32, P]
+        for n in ("moracle_macros", "moracle_get_pops", "moracle_flags"):
+            getattr(L, n).argtypes = [V, C.c_uint32, P]
+        for n in ("moracle_records", "moracle_vertex_means", "moracle_slice_means"):
+            getattr(L, n).argtypes = [V, P]
+        L.moracle_reset_samples.argtypes = [V]
+        L.moracle_gauss_f64.argtypes = [C.c_double, P, P, P, P]
+        L.moracle_feq_f64.argtypes = [C.c_double, P, P]
+        L.moracle_rebuild_f64.argtypes = [C.c_double, P, P, P]
+        L.moracle_collide_f64.argtypes = [P, C.c_double, C.c_double, P]
+        _mlib_done = True
+    return L
+
+
+class MultiOracle:
+    """CPU restatement of `MultiLattice` (mlattice.rs:139-948). store: F32,
+    F16 (the reference's per-store rounding) or F64."""
+
+    def __init__(self, setup, geometry=None, probes=None, store: int = F32, threads: int = 0):
+        L = mlib()
+        self._lib = L
+        self.setup = setup
+        self.store = store
+        cs = setup.to_c()
+        g, keep_g = sa.geometry_to_c(geometry)
+        p, keep_p = sa.probes_to_c(probes)
+        nthreads = threads or min(os.cpu_count() or 1, 16)
+        self._h = L.moracle_new(C.byref(cs), C.byref(g), C.byref(p) if p is not None else None, int(store), nthreads)
+        del keep_g, keep_p
+        if not self._h:
+            raise sa.SimlabError(L.moracle_last_error().decode())
+        self.nvert = len(probes.verts) if probes else 0
+        self.npix = int(L.moracle_npix(self._h))
+        self.nrec = int(L.moracle_nrec(self._h))
+        nl = int(L.moracle_levels(self._h))
+        d = np.zeros(3 * nl, np.uint32)
+        L.moracle_level_dims(self._h, _ptr(d))
+        self.dims = [tuple(int(v) for v in d[3 * l:3 * l + 3]) for l in range(nl)]
+        self.cells = [int(np.prod(x)) for x in self.dims]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.moracle_free(h)
+            self._h = None
+
+    def sub_steps(self):
+        return 1 << (len(self.dims) - 1)
+
+    def run(self, steps):
+        out = np.zeros((len(steps), 6), np.float64)
+        arr = sa.steps_to_c(steps)
+        if self._lib.moracle_run(self._h, arr, len(steps), _ptr(out)) != 0:
+            raise sa.SimlabError(self._lib.moracle_last_error().decode())
+        return out
+
+    def run_still(self, n):
+        out = np.zeros((int(n), 6), np.float64)
+        if self._lib.moracle_run_still(self._h, int(n), _ptr(out)) != 0:
+            raise sa.SimlabError(self._lib.moracle_last_error().decode())
+        return out
+
+    def macros(self, level):
+        out = np.zeros((self.cells[level], 4), np.float32)
+        self._lib.moracle_macros(self._h, int(level), _ptr(out))
+        return out
+
+    def get_pops(self, level):
+        out = np.zeros((27, self.cells[level]), np.float64)
+        self._lib.moracle_get_pops(self._h, int(level), _ptr(out))
+        return out
+
+    def set_pops(self, level, pops):
+        a = np.ascontiguousarray(pops, np.float64).reshape(27, self.cells[level])
+        self._lib.moracle_set_pops(self._h, int(level), _ptr(a))
+
+    def flags(self, level):
+        out = np.zeros(self.cells[level], np.uint32)
+        self._lib.moracle_flags(self._h, int(level), _ptr(out))
+        return out
+
+    def records(self):
+        out = np.zeros((max(self.nrec, 1), 24), np.uint32)
+        self._lib.moracle_records(self._h, _ptr(out))
+        return out
+
+    def reset_samples(self):
+        self._lib.moracle_reset_samples(self._h)
+
+    def vertex_means(self):
+        out = np.zeros(self.nvert, np.float32)
+        self._lib.moracle_vertex_means(self._h, _ptr(out))
+        return out
+
+    def slice_means(self):
+        out = np.zeros((self.npix, 4), np.float32)
+        self._lib.moracle_slice_means(self._h, _ptr(out))
+        return out
+
+
+def gauss27_f64(rho, u, sg, m1=(0.0, 0.0, 0.0)):
+    """The oracle's Gaussian post-collision state (27 shifted populations)."""
+    a = [np.ascontiguousarray(v, np.float64) for v in (u, sg, m1)]
+    out = np.zeros(27)
+    mlib().moracle_gauss_f64(float(rho), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(out))
+    return out
+
+
+def feq27_f64(rho, u):
+    a = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(27)
+    mlib().moracle_feq_f64(float(rho), _ptr(a), _ptr(out))
+    return out
+
+
+def rebuild27_f64(rho, u, p):
+    a = [np.ascontiguousarray(v, np.float64) for v in (u, p)]
+    out = np.zeros(27)
+    mlib().moracle_rebuild_f64(float(rho), _ptr(a[0]), _ptr(a[1]), _ptr(out))
+    return out
+
+
+def collide27_f64(f, tau0, smag2=0.0):
+    a = np.ascontiguousarray(f, np.float64)
+    out = np.zeros(27)
+    mlib().moracle_collide_f64(_ptr(a), float(tau0), float(smag2), _ptr(out))
+    return out

@@ -339,6 +339,11 @@ def voxelize_static(verts, tris, dims, gpu: bool = False) -> StaticVoxels:
     h = C.c_void_p()
     fn = lib.simlab_cfd_voxelize_static_gpu if gpu else lib.simlab_cfd_voxelize_static
     _check(lib, fn(_ptr(v), len(v), _ptr(t), len(t), d, C.byref(h)))
+    return _voxels_from_handle(lib, h)
+
+
+def _voxels_from_handle(lib, h) -> StaticVoxels:
+    """Copies a simlab_static_voxels into a StaticVoxels and frees it."""
     try:
         ns, nw = C.c_uint64(), C.c_uint64()
         ps, pw = C.c_void_p(), C.c_void_p()

@@ -56,6 +56,10 @@ def build(verbose=False, save_temps=False):
         if save_temps:
             cmd.insert(1, "--save-temps")
         jobs.append((cmd, obj))
+    obj = os.path.join(OUT, "mkernels.o")
+    jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "mkernels.hip"), "-o", obj], obj))
+    obj = os.path.join(OUT, "mlattice.cpp.o")
+    jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "mlattice.cpp"), "-o", obj], obj))
     obj = os.path.join(OUT, "voxel_gpu.o")
     jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "voxel_gpu.hip"), "-o", obj], obj))
     obj = os.path.join(OUT, "lattice.cpp.o")

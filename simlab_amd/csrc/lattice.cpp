@@ -3,6 +3,7 @@
 // per-step launch sequence (lattice.rs:699-753) and the read-backs
 // (lattice.rs:755-841), behind the C ABI of include/simlab_cfd.h.
 
+#include "hip_util.h"
 #include "kernels.h"
 #include "simlab_internal.h"
 
@@ -25,19 +26,6 @@ namespace simlab {
 namespace {
 thread_local std::string g_err;
 
-struct HipError : std::runtime_error {
-    int code;
-    HipError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) {
-        (void)hipGetLastError();   // clear the sticky error of failed launches
-        throw HipError(e == hipErrorOutOfMemory ? SIMLAB_ERR_OOM : SIMLAB_ERR_GPU,
-                       std::string(what) + ": " + hipGetErrorString(e));
-    }
-}
-
 size_t elem_bytes(int storage) {
     switch (storage) {
         case SIMLAB_STORAGE_FP32: return 4;
@@ -46,19 +34,6 @@ size_t elem_bytes(int storage) {
     }
 }
 size_t real_bytes(int storage) { return storage == SIMLAB_STORAGE_FP64 ? 8 : 4; }
-
-uint16_t f16_bits_host(float x) {   // lattice.rs:172-189
-    uint32_t b;
-    std::memcpy(&b, &x, 4);
-    uint16_t sign = (uint16_t)((b >> 16) & 0x8000);
-    int exp = (int)((b >> 23) & 0xff) - 127 + 15;
-    uint32_t man = b & 0x7fffff;
-    if (exp <= 0) return sign;
-    if (exp >= 31) return sign | 0x7c00;
-    uint32_t h = sign | ((uint32_t)exp << 10) | (man >> 13);
-    if (man & 0x1000) h += 1;
-    return (uint16_t)h;
-}
 
 float half_to_float(uint16_t h) {
     uint32_t s = (uint32_t)(h & 0x8000) << 16, e = (h >> 10) & 0x1f, m = h & 0x3ff;
@@ -71,29 +46,6 @@ float half_to_float(uint16_t h) {
     std::memcpy(&v, &bits, 4);
     return v;
 }
-
-// Device buffers owned by a lattice; freed in the destructor.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void alloc(size_t n, size_t &total) {
-        bytes = std::max<size_t>(n, 16);
-        hip_check(hipMalloc(&p, bytes), "hipMalloc");
-        total += bytes;
-    }
-    template <class T> T *as() const { return (T *)p; }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
 
 }  // namespace
 

@@ -6,6 +6,7 @@
 #include <array>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <exception>
 #include <new>
 #include <string>
@@ -21,6 +22,21 @@ constexpr int CV[19][3] = {{0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0,
                            {1, 0, -1},  {-1, 0, 1},  {0, 1, -1}, {0, -1, 1}};
 constexpr int opp(int i) { return i == 0 ? 0 : (i % 2 == 1 ? i + 1 : i - 1); }
 constexpr double weight(int i) { return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+// lattice.rs:172-189 (mlattice.rs:146-163): host rounding of a wall q to
+// half (add bit 12, flush subnormals to signed zero)
+inline uint16_t f16_bits_host(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    uint16_t sign = (uint16_t)((b >> 16) & 0x8000);
+    int exp = (int)((b >> 23) & 0xff) - 127 + 15;
+    uint32_t man = b & 0x7fffff;
+    if (exp <= 0) return sign;
+    if (exp >= 31) return sign | 0x7c00;
+    uint32_t h = sign | ((uint32_t)exp << 10) | (man >> 13);
+    if (man & 0x1000) h += 1;
+    return (uint16_t)h;
+}
 
 struct Surface {
     std::vector<std::array<double, 3>> verts;
@@ -51,6 +67,8 @@ StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]);
 // the same voxelization with the inside test and the link search on the GPU
 // (the current device); identical output
 StaticVoxels voxelize_static_gpu(const Surface &s, const size_t dims[3]);
+// the same voxelization with the D3Q27 links (voxel.rs:354-359)
+StaticVoxels voxelize_static27(const Surface &s, const size_t dims[3]);
 Surface subdivide(const Surface &s, double max_edge);
 SpinVoxels voxelize_spin(const Surface &s, const double axis[3], double h);
 

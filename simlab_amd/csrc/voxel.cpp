@@ -385,12 +385,44 @@ bool inside_grid_gpu(const Surface &s, const Grid &g, const DMem &d_tris, DMem &
 }
 }  // namespace
 
-// voxel.rs:348-352 + 363-490 (D3Q19 links). gpu: the inside test and the
-// link search on the GPU (the same decisions; voxel_gpu.hip)
-StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool gpu) {
+// A lattice velocity set for the link search: nd directions, their
+// velocities and opposites, and the rest direction (skipped).
+struct LinkSet {
+    int nd;
+    int rest;
+    int c[27][3];
+    int opp[27];
+};
+LinkSet d3q19_links() {
+    LinkSet v{19, 0, {}, {}};
+    for (int d = 0; d < 19; ++d) {
+        for (int k = 0; k < 3; ++k) v.c[d][k] = CV[d][k];
+        v.opp[d] = opp(d);
+    }
+    return v;
+}
+// d3q27.rs:5-15: direction i = (i % 3 - 1, (i / 3) % 3 - 1, i / 9 - 1), rest 13, opposite 26 - i
+LinkSet d3q27_links() {
+    LinkSet v{27, 13, {}, {}};
+    for (int d = 0; d < 27; ++d) {
+        v.c[d][0] = d % 3 - 1;
+        v.c[d][1] = (d / 3) % 3 - 1;
+        v.c[d][2] = d / 9 - 1;
+        v.opp[d] = 26 - d;
+    }
+    return v;
+}
+
+// voxel.rs:348-359 + 363-490 (voxelize_links for the D3Q19 or D3Q27 set).
+// gpu (D3Q19 only): the inside test and the link search on the GPU (the
+// same decisions; voxel_gpu.hip)
+StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool gpu, const LinkSet &ls) {
     StaticVoxels out;
-    const int nd = 19, rest = 0;
-    const double maxlen = std::sqrt(2.0);
+    const int nd = ls.nd, rest = ls.rest;
+    double maxlen = 0.0;
+    for (int d = 0; d < nd; ++d)
+        maxlen = std::max(maxlen, std::sqrt((double)(ls.c[d][0] * ls.c[d][0] + ls.c[d][1] * ls.c[d][1] + ls.c[d][2] * ls.c[d][2])));
+    if (nd != 19) gpu = false;
     P3 blo = {INFINITY, INFINITY, INFINITY}, bhi = {-INFINITY, -INFINITY, -INFINITY};
     for (const auto &p : s.verts)
         for (int k = 0; k < 3; ++k) {
@@ -533,7 +565,7 @@ StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool g
             for (float &q : cell.q) q = 1.0f;
             for (int d = 0; d < nd; ++d) {
                 if (d == rest) continue;
-                const int *c = CV[d];
+                const int *c = ls.c[d];
                 int64_t ni = (int64_t)i + c[0], nj = (int64_t)j + c[1], nk = (int64_t)k + c[2];
                 double dv[3] = {(double)c[0], (double)c[1], (double)c[2]};
                 double tmin = INFINITY, tmax = -INFINITY;
@@ -558,7 +590,7 @@ StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool g
                     cell.q[d] = (float)std::min(std::max(tmin, 1e-3), 1.0);
                     cell.mask |= 1u << d;
                     if (in_region)
-                        cell.thin.push_back({g.idx((size_t)ni, (size_t)nj, (size_t)nk), opp(d),
+                        cell.thin.push_back({g.idx((size_t)ni, (size_t)nj, (size_t)nk), ls.opp[d],
                                              (float)std::min(std::max(1.0 - tmax, 1e-3), 1.0)});
                 }
             }
@@ -610,8 +642,15 @@ StaticVoxels voxelize_static_impl(const Surface &s, const size_t dims[3], bool g
     return out;
 }
 
-StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) { return voxelize_static_impl(s, dims, false); }
-StaticVoxels voxelize_static_gpu(const Surface &s, const size_t dims[3]) { return voxelize_static_impl(s, dims, true); }
+StaticVoxels voxelize_static(const Surface &s, const size_t dims[3]) {
+    return voxelize_static_impl(s, dims, false, d3q19_links());
+}
+StaticVoxels voxelize_static_gpu(const Surface &s, const size_t dims[3]) {
+    return voxelize_static_impl(s, dims, true, d3q19_links());
+}
+StaticVoxels voxelize_static27(const Surface &s, const size_t dims[3]) {
+    return voxelize_static_impl(s, dims, false, d3q27_links());
+}
 
 // voxel.rs:562-584: split triangles 1 -> 4 until no edge exceeds max_edge
 Surface subdivide(const Surface &s, double max_edge) {
@@ -720,6 +759,20 @@ int simlab_cfd_voxelize_static(const double *verts, uint64_t nverts, const uint3
         size_t d[3] = {(size_t)dims[0], (size_t)dims[1], (size_t)dims[2]};
         auto *v = new simlab_static_voxels;
         v->v = simlab::voxelize_static(s, d);
+        *out = v;
+        return SIMLAB_OK;
+    });
+}
+
+int simlab_cfd_voxelize_static27(const double *verts, uint64_t nverts, const uint32_t *tris, uint64_t ntris,
+                                 const uint64_t dims[3], simlab_static_voxels **out) {
+    return simlab::guarded([&]() -> int {
+        if (!out || !dims || (nverts && !verts) || (ntris && !tris)) return simlab::fail(SIMLAB_ERR_INVALID, "null argument");
+        simlab::Surface s;
+        if (!make_surface(verts, nverts, tris, ntris, s)) return simlab::fail(SIMLAB_ERR_INVALID, "triangle index out of range");
+        size_t d[3] = {(size_t)dims[0], (size_t)dims[1], (size_t)dims[2]};
+        auto *v = new simlab_static_voxels;
+        v->v = simlab::voxelize_static27(s, d);
         *out = v;
         return SIMLAB_OK;
     });

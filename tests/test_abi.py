@@ -56,3 +56,29 @@ def test_struct_sizes_match_the_header():
 
 def test_max_cells_is_positive():
     assert sa.max_cells(sa.Storage.FP16) >= 1 << 30
+
+
+def test_refined_setup_errors_carry_the_reference_messages():
+    """mlattice.rs:192-209 and 713-717, checked before any device work."""
+    from simlab_amd import mlattice as ml
+    F = sa.FaceBc
+    s = ml.MultiSetup(dims0=(32, 24, 24), faces=(F.PERIODIC,) * 6, boxes=[ml.LevelBox((8, 6, 6), (16, 12, 12))])
+    with pytest.raises(sa.SimlabError, match="periodic faces only work without refinement"):
+        ml.MultiLattice(s)
+    s.faces = (F.VELOCITY,) + (F.OPEN,) * 5
+    s.boxes = [ml.LevelBox((1, 6, 6), (16, 12, 12))]
+    with pytest.raises(sa.SimlabError, match="refinement box 1 does not fit inside its parent"):
+        ml.MultiLattice(s)
+    s.boxes = [ml.LevelBox((8, 6, 6), (16, 12, 7))]
+    with pytest.raises(sa.SimlabError, match="refinement box 1 does not fit inside its parent"):
+        ml.MultiLattice(s)
+    s.boxes = []
+    s.storage = sa.Storage.BF16
+    with pytest.raises(sa.SimlabError, match="FP32, FP16 or FP64"):
+        ml.MultiLattice(s)
+
+
+def test_refined_struct_layout():
+    from simlab_amd import mlattice as ml
+    assert C.sizeof(ml._LevelBox) == 24
+    assert C.sizeof(ml._MultiSetup) == 16 + 7 * 24 + 24 + 8 * 2 + 24 * 3 + 24 + 16 + 16 + 24
