@@ -201,11 +201,19 @@ class MultiLattice:
 
     def __del__(self):
         try:
-            if self._h:
-                self._lib.simlab_cfd_multi_free(self._h)
-                self._h = C.c_void_p()
+            self.close()
         except Exception:
             pass
+
+    def close(self):
+        """Frees the device buffers now (the Rust `Drop`)."""
+        if self._h:
+            self._lib.simlab_cfd_multi_free(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def storage(self) -> Storage:
+        return Storage(self.info().storage)
 
     def info(self) -> _MultiInfo:
         i = _MultiInfo()

@@ -1,11 +1,11 @@
-"""The study driver over the lattice: a restatement of crates/cfd/src/tunnel.rs
-(`plan`, `estimate`, `finest_fit`, `run_with`, `sig`) for the uniform D3Q19
-grid, and the reference cases of crates/cfd/src/validation.rs.
+"""The study driver over the lattices: a restatement of crates/cfd/src/tunnel.rs
+(`plan`, `estimate`, `finest_fit`, `run_with`, `sig`) and the reference
+cases of crates/cfd/src/validation.rs.
 
-The reference runs still parts of 48 or more cells across on its refined
-D3Q27 solver (tunnel.rs:631-676, `SolverOptions::refine`); that solver is not
-part of this build, so every study here steps one uniform D3Q19 lattice (as
-the reference does with `refine = false`, and for every spinning part).
+As in the reference, a still part with 48 or more cells across its largest
+size runs on the refined D3Q27 solver (`MultiLattice`, nested boxes round
+the part, tunnel.rs:645-687, `SolverOptions.refine`); spinning parts and
+coarser still parts step one uniform D3Q19 lattice.
 
 Units: the surface is in metres; `unit_m` converts the spin centre (model
 units, e.g. mm) to metres; results are SI (the conventions of units.py).
@@ -22,6 +22,7 @@ import numpy as np
 from . import (FaceBc, Geometry, Lattice, LatticeSetup, Probes, SliceDef, StepIn, Storage, voxelize_spin,
                voxelize_static)
 from . import units as U
+from .mlattice import LevelBox, MultiLattice, MultiSetup, voxelize_static27
 from .shapes import TriMesh, disk, sphere, volume
 from .workloads import rotation
 
@@ -29,9 +30,22 @@ from .workloads import rotation
 # and its per-call batch size, which also sets when convergence is checked)
 MLUPS_FP32 = 2800.0
 MLUPS_FP16 = 4500.0
+MLUPS_REFINED = 1400.0     # the refined D3Q27 solver, all levels
 THIN_MAX_BOOST = 8
 SIDE_MIN_FRONT = 1.5
 REFINE_THIN = 2.0
+# tunnel.rs:462-466: nested refinement round the part beyond its bounding
+# box, in frontal sizes: (upstream, downstream, sideways) for levels 1, 2, 3
+REFINE_EXT = ((1.2, 3.0, 1.0), (0.8, 1.6, 0.75), (0.65, 1.0, 0.65))
+
+
+def still_levels(cells_across: int) -> int:
+    """tunnel.rs:467-475: levels for a still part; the coarsest keeps at
+    least 24 cells across the part's largest size, at most 4 levels."""
+    nl = 1
+    while nl < 4 and cells_across / float(1 << nl) >= 24.0:
+        nl += 1
+    return nl
 
 
 # ---------------------------------------------------------------- configuration (simlab_core config.rs)
@@ -86,6 +100,7 @@ class SolverOptions:
     absorb_velocity: bool = False
     total_pressure_inlet: bool = True
     wall_model: bool = False
+    refine: bool = True
 
 
 def cells_across(q) -> int:
@@ -248,12 +263,23 @@ class Plan:
     updates_per_step: float
     d_thin: float
     thin: Optional[tuple] = None    # (target, wanted) when a slender part's grid was sized for its thickness
+    # refined grid (still parts): coarsest dims, nested boxes, the finest
+    # level's origin in global finest cells, cells per level (tunnel.rs:455-460)
+    multi: Optional["MultiPlan"] = None
 
     def cost(self) -> float:
         return self.updates_per_step * math.ceil(self.duration * self.steps_unit)
 
     def cells_thin(self) -> float:
         return self.d_thin / self.dx
+
+
+@dataclass
+class MultiPlan:
+    dims0: tuple
+    boxes: list
+    org_f: tuple
+    cells: list
 
 
 def _bytes_per_cell(s: Storage) -> int:
@@ -340,6 +366,41 @@ def plan(cfg: WindConfig, m: TriMesh, unit_m: float, opt: SolverOptions, budget:
     dom_hi = np.array([hi[0] + mg.downstream * d_front, hi[1] + side, hi[2] + side])
     dims = tuple(int(math.ceil((dom_hi[k] - dom_lo[k]) / dx)) + 1 for k in range(3))
     center = tuple(float(-dom_lo[k] / dx) for k in range(3))
+    # still parts: nested refinement, the finest level holding the part (tunnel.rs:643-687)
+    multi = None
+    if spin is None and opt.refine:
+        nl = still_levels(n_across)
+        if nl > 1:
+            s0 = float(1 << (nl - 1))
+            dx0 = dx * s0
+            dims0 = tuple(int(math.ceil((dom_hi[k] - dom_lo[k]) / dx0)) + 1 for k in range(3))
+            dims = tuple(int((dims0[k] - 1) * s0) + 1 for k in range(3))
+            lo_c = [(center[k] + lo[k] / dx) / s0 for k in range(3)]
+            hi_c = [(center[k] + hi[k] / dx) / s0 for k in range(3)]
+            pd = list(dims0)
+            boxes = []
+            org = [0.0, 0.0, 0.0]
+            cells = [int(np.prod(np.asarray(dims0, np.int64)))]
+            for l in range(1, nl):
+                sc = float(1 << (nl - l))          # finest cells per cell of level l - 1
+                df = d_box / dx / sc
+                eu, ed, es = REFINE_EXT[l - 1]
+                e_lo = (eu * df, es * df, es * df)
+                e_hi = (ed * df, es * df, es * df)
+                blo, bn = [0] * 3, [0] * 3
+                for k in range(3):
+                    a0 = max(int(math.floor(lo_c[k] - e_lo[k])), 3)
+                    b0 = min(int(math.ceil(hi_c[k] + e_hi[k])), pd[k] - 4)
+                    blo[k] = a0
+                    bn[k] = max(b0 - a0 + 1, 8)
+                for k in range(3):
+                    org[k] += blo[k] * sc
+                    lo_c[k] = 2.0 * (lo_c[k] - blo[k])
+                    hi_c[k] = 2.0 * (hi_c[k] - blo[k])
+                    pd[k] = 2 * (bn[k] - 1) + 1
+                cells.append(int(np.prod(np.asarray(pd, np.int64))))
+                boxes.append(LevelBox(lo=tuple(blo), n=tuple(bn)))
+            multi = MultiPlan(dims0=dims0, boxes=boxes, org_f=tuple(org), cells=cells)
     # time step from the Mach limit, then the viscosity limit (tunnel.rs:690-700)
     omega = spin.rpm * 2.0 * math.pi / 60.0 if spin else 0.0
     tip = abs(omega) * rotor_radius
@@ -372,20 +433,30 @@ def plan(cfg: WindConfig, m: TriMesh, unit_m: float, opt: SolverOptions, budget:
     bud = budget if budget is not None else opt.memory_budget
 
     def mem_for(s):
+        if multi is not None:
+            # A and B population sets, flags and moments per level (tunnel.rs:726-733)
+            pops = sum(c * (2 * 27 * s.pop_bytes() + 4 + 8) for c in multi.cells)
+            return pops + est_records * 24 * 4 * 2 + 64 * 1024 * 1024
         return n * _bytes_per_cell(s) + est_records * 40 * 4 + faces_cells * 44 + 64 * 1024 * 1024
 
+    # the refined solver keeps 32-bit cell indices per level
+    n_lay = max(multi.cells) if multi is not None else n
+    lay_max = (1 << 32) - 2 if multi is not None else max_cells_layout(Storage.FP32)
     if opt.storage is not None:
         storage = opt.storage
-    elif (bud is not None and mem_for(Storage.FP32) > bud) or n > max_cells_layout(Storage.FP32):
+    elif (bud is not None and mem_for(Storage.FP32) > bud) or n_lay > lay_max:
         storage = Storage.FP16
     else:
         storage = Storage.FP32
-    memory = mem_for(storage) if n <= max_cells_layout(storage) else (1 << 62)
+    memory = mem_for(storage) if n_lay <= lay_max else (1 << 62)
     frontal = math.pi * rotor_radius ** 2 if still else frontal_area(m, r)
     blockage = frontal / ((dom_hi[1] - dom_lo[1]) * (dom_hi[2] - dom_lo[2]))
 
+    # sponge widths in cells of the coarsest level (tunnel.rs:757-761)
+    dxs = dx * ((dims[0] - 1) / (multi.dims0[0] - 1)) if multi is not None else dx
+
     def cl(x, least):
-        return max(int(round(x / dx)), least)
+        return max(int(round(x / dxs)), least)
 
     if still:
         s = cl(opt.sponge_widths[2] * d_front, 8)
@@ -396,12 +467,18 @@ def plan(cfg: WindConfig, m: TriMesh, unit_m: float, opt: SolverOptions, budget:
         s_side = cl(opt.sponge_widths[2] * d_front, 6)
         sponge = (cl(opt.sponge_widths[0] * d_front, 6), cl(opt.sponge_widths[1] * d_front, 8), s_side, s_side, s_side, s_side)
         faces = (FaceBc.VELOCITY, FaceBc.OPEN, sbc, sbc, sbc, sbc)
+    # cell updates per finest step, all levels (tunnel.rs:800-806)
+    if multi is not None:
+        nl = len(multi.cells)
+        ups = sum(c * float(1 << l) for l, c in enumerate(multi.cells)) / float(1 << (nl - 1))
+    else:
+        ups = float(n)
     return Plan(r=r, pref=pref, dx=dx, dt=dt, dims=dims, center=center, u_lat=u_lat, omega_lat=omega_lat, nu_lat=nu_lat,
                 spin=spin is not None, still=still, cells_across=n_across, d_front=d_front, frontal_area=frontal,
                 steps_unit=steps_unit, duration=duration, storage=storage, memory=memory, blockage=blockage,
                 sponge=sponge, faces=faces, rotor_radius=rotor_radius, axis_lat=axis_lat, axis_model=axis_model,
-                box_lo=box_lo, box_n=box_n, est_records=est_records, updates_per_step=float(n), d_thin=d_thin,
-                thin=thin)
+                box_lo=box_lo, box_n=box_n, est_records=est_records, updates_per_step=ups, d_thin=d_thin,
+                thin=thin, multi=multi)
 
 
 def _thin_warning(p: Plan) -> Optional[str]:
@@ -441,11 +518,15 @@ def estimate(cfg: WindConfig, m: TriMesh, unit_m: float, budget_bytes: int, opt:
     """tunnel.rs:815-866."""
     opt = opt or SolverOptions()
     p = plan(cfg, m, unit_m, opt, budget_bytes)
-    cells = int(np.prod(np.asarray(p.dims, np.int64)))
+    cells = sum(p.multi.cells) if p.multi is not None else int(np.prod(np.asarray(p.dims, np.int64)))
     steps = int(math.ceil(p.duration * p.steps_unit))
-    mlups = MLUPS_FP16 if p.storage == Storage.FP16 else MLUPS_FP32
+    if p.multi is not None:
+        mlups = MLUPS_REFINED
+    else:
+        mlups = MLUPS_FP16 if p.storage == Storage.FP16 else MLUPS_FP32
     seconds = steps * p.updates_per_step / (mlups * 1e6) * (1.15 if p.spin else 1.0) + 2.0 + p.est_records * 2e-6
-    note = (f"{p.cells_across} cells across the part ({p.dx * 1e3:.2f} mm cells), {p.dims[0]} x {p.dims[1]} x {p.dims[2]} "
+    levels = f" near it, {len(p.multi.cells)} grid levels" if p.multi is not None else ""
+    note = (f"{p.cells_across} cells across the part ({p.dx * 1e3:.2f} mm cells{levels}), {p.dims[0]} x {p.dims[1]} x {p.dims[2]} "
             f"grid, {'FP16' if p.storage == Storage.FP16 else 'FP32'} storage, {p.duration:.0f} "
             f"{'revolutions' if p.still else 'flow passes'}")
     return Estimate(fits=p.memory <= budget_bytes, cells=cells, grid=p.dims, cells_across=p.cells_across,
@@ -666,6 +747,13 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         solid = float(sum(bin(int(b)).count("1") for b in vox.occ_bits)) * vox.occ_h ** 3
         _check_thin(warnings, solid, surface, p.dx)
         geom = Geometry.spin(vox, p.box_lo, p.box_n)
+    elif p.multi is not None:
+        # the finest level's cells, in its own coordinates (tunnel.rs:929-938)
+        b = p.multi.boxes[-1]
+        fd = tuple(2 * (int(b.n[k]) - 1) + 1 for k in range(3))
+        vox = voxelize_static27(np.asarray(p.center) + to_body - np.asarray(p.multi.org_f), surface.triangles, fd)
+        _check_thin(warnings, float(len(vox.solid)), surface, p.dx)
+        geom = Geometry.static(vox)
     else:
         vox = voxelize_static(np.asarray(p.center) + to_body, surface.triangles, p.dims)
         _check_thin(warnings, float(len(vox.solid)), surface, p.dx)
@@ -675,18 +763,48 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
     flip = -1.0 if volume(surface) < 0.0 else 1.0
     normals = (_vertex_normals(surface) * flip) @ p.r.T
     nx, ny, nz = p.dims
-    xc = max(int(round(min(max(p.center[0], 1.0), nx - 2.0))), 1)
-    yc = max(int(round(min(max(p.center[1], 1.0), ny - 2.0))), 1)
-    slices = [SliceDef((1, yc, 1), (1, 0, 0), (0, 0, 1), max(nx - 2, 1), max(nz - 2, 1)),
-              SliceDef((xc, 1, 1), (0, 1, 0), (0, 0, 1), max(ny - 2, 1), max(nz - 2, 1))]
+    if p.multi is not None:
+        # slices through the part centre in global finest cells; each pixel
+        # from the finest level covering it, with a pitch that keeps a slice
+        # under ~4 M pixels (tunnel.rs:966-989)
+        pitch = 1
+        while max((nx // pitch) * (nz // pitch), (ny // pitch) * (nz // pitch)) > 4_000_000:
+            pitch *= 2
+
+        def snap(x):
+            return max(int(round(x / pitch)) * pitch, pitch)
+
+        def span(n):
+            return max((n - 1 - pitch) // pitch, 1)
+
+        xc = snap(min(max(p.center[0], 1.0), nx - 2.0))
+        yc = snap(min(max(p.center[1], 1.0), ny - 2.0))
+        slices = [SliceDef((pitch, yc, pitch), (pitch, 0, 0), (0, 0, pitch), span(nx), span(nz)),
+                  SliceDef((xc, pitch, pitch), (0, pitch, 0), (0, 0, pitch), span(ny), span(nz))]
+    else:
+        xc = max(int(round(min(max(p.center[0], 1.0), nx - 2.0))), 1)
+        yc = max(int(round(min(max(p.center[1], 1.0), ny - 2.0))), 1)
+        slices = [SliceDef((1, yc, 1), (1, 0, 0), (0, 0, 1), max(nx - 2, 1), max(nz - 2, 1)),
+                  SliceDef((xc, 1, 1), (0, 1, 0), (0, 0, 1), max(ny - 2, 1), max(nz - 2, 1))]
     probes = Probes([(tuple(b), tuple(nv)) for b, nv in zip(to_body, normals)], slices, opt.vertex_offset)
     u_inf = (p.u_lat, 0.0, 0.0)
-    setup = LatticeSetup(dims=p.dims, faces=p.faces, nu=p.nu_lat, smagorinsky=opt.smagorinsky, u_inf=u_inf,
-                         u_init=u_inf, sponge=p.sponge, sponge_tau=1.0, sponge_sigma=opt.sponge_sigma,
-                         absorb_velocity=opt.absorb_velocity and not p.still,
-                         total_pressure_inlet=opt.total_pressure_inlet, storage=p.storage, center=p.center,
-                         wall_model=opt.wall_model)
-    lat = Lattice(setup, geom, probes)
+    if p.multi is not None:
+        nl = len(p.multi.cells)
+        msetup = MultiSetup(dims0=p.multi.dims0, boxes=list(p.multi.boxes), faces=p.faces,
+                            nu0=p.nu_lat / float(1 << (nl - 1)), smagorinsky=0.0, u_inf=u_inf, u_init=u_inf,
+                            sponge=p.sponge, sponge_tau=1.0, sponge_sigma=opt.sponge_sigma,
+                            absorb_velocity=opt.absorb_velocity, total_pressure_inlet=opt.total_pressure_inlet,
+                            storage=p.storage,
+                            center=tuple(p.center[k] - p.multi.org_f[k] for k in range(3)))
+        lat = MultiLattice(msetup, geom, probes)
+    else:
+        setup = LatticeSetup(dims=p.dims, faces=p.faces, nu=p.nu_lat, smagorinsky=opt.smagorinsky, u_inf=u_inf,
+                             u_init=u_inf, sponge=p.sponge, sponge_tau=1.0, sponge_sigma=opt.sponge_sigma,
+                             absorb_velocity=opt.absorb_velocity and not p.still,
+                             total_pressure_inlet=opt.total_pressure_inlet, storage=p.storage, center=p.center,
+                             wall_model=opt.wall_model)
+        lat = Lattice(setup, geom, probes)
+    ss = lat.sub_steps() if p.multi is not None else 1
     t_setup = time.perf_counter() - t_start
     try:
         if lat.storage != p.storage:
@@ -694,7 +812,7 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         if lat.storage == Storage.FP16:
             warnings.append("FP16 storage (to fit the memory budget): slightly noisier forces")
 
-        total_steps = int(math.ceil(p.duration * p.steps_unit))
+        total_steps = -(-int(math.ceil(p.duration * p.steps_unit)) // ss) * ss
         omega_mag = _norm(p.omega_lat)
         ramp_steps = int(math.ceil(opt.spin_ramp_revs * 2.0 * math.pi / omega_mag)) if p.spin else 0
         unit_steps = p.steps_unit
@@ -706,7 +824,7 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         min_win = int(min(min_win_units, 0.3 * p.duration) * unit_steps)
         w0 = min(w0, total_steps // 2)
         sample_every = max(int(math.floor(unit_steps / 100.0)), 1)
-        batch = min(max(int(0.25 * MLUPS_FP32 * 1e6 / p.updates_per_step), 16), 4096)
+        batch = -(-min(max(int(0.25 * MLUPS_FP32 * 1e6 / p.updates_per_step), 16), 4096) // ss) * ss
 
         hist = np.zeros((total_steps, 6))
         theta = 0.0
@@ -855,7 +973,8 @@ def run_with(cfg: WindConfig, surface: TriMesh, unit_m: float, opt: Optional[Sol
         summary = WindSummary(
             headline=headline, force_n=s["force_n"], moment_nm=s["moment_nm"], drag_n=drag, lift_n=s["lift_n"],
             side_n=s["side_n"], cd=s["cd"], ref_area_mm2=p.frontal_area * 1e6, thrust_n=thrust, torque_nm=torque,
-            power_w=power, reynolds=reynolds, cells=int(np.prod(np.asarray(p.dims, np.int64))), steps=step,
+            power_w=power, reynolds=reynolds,
+            cells=sum(p.multi.cells) if p.multi is not None else int(np.prod(np.asarray(p.dims, np.int64))), steps=step,
             mlups=mlups, seconds=time.perf_counter() - t_start, converged=converged, warnings=warnings, grid=p.dims,
             cells_across=p.cells_across, dx_mm=p.dx * 1e3, dt_s=p.dt,
             mach_lattice=(p.u_lat + omega_mag * p.rotor_radius / p.dx) * math.sqrt(3.0), tau=tau0,

@@ -3,9 +3,9 @@ crates/cfd/tests/tunnel.rs runs them through `run_with`: the sphere at
 Re 100 on 16 cells across against the drag correlations, the test propeller
 in hover (thrust along +axis, torque against the rotation, shaft power
 positive, for either spin direction) and a slowly spinning sphere against
-Stokes' torque. Every run here is on the uniform D3Q19 grid, which is the
-grid the reference uses for these cases too (under 48 cells across, and for
-every spinning part)."""
+Stokes' torque on the uniform D3Q19 grid (under 48 cells across, and for
+every spinning part, as in the reference), and the spheres of 48 cells
+across on the refined D3Q27 grid, as the reference plans them."""
 import math
 
 import numpy as np
@@ -83,13 +83,32 @@ def test_sphere_re20():
     assert abs(cd / REF_CD_RE20 - 1.0) < 0.005
 
 
-@pytest.mark.parametrize("re,d,passes,tol", [(100.0, 48, 30.0, 0.03), (300.0, 48, 40.0, 0.06)])
-def test_sphere_on_48_cells(re, d, passes, tol):
+def test_refined_sphere_re100_quick():
+    # crates/cfd/tests/refined.rs:122-131: 48 cells across, two levels (24
+    # on the coarse grid), 12 passes, within 3 % of the correlations
+    _say("study: refined sphere Re 100, 48 cells across, 12 passes")
+    cd, r = T.sphere_drag(100.0, 48, 12.0)
+    s = r.summary
+    refc = 0.5 * (T.sphere_cd_cgw(100.0) + T.sphere_cd_sn(100.0))
+    _say(f"  Cd {cd:.4f} vs {refc:.4f} ({(cd / refc - 1) * 100:+.2f} %), {s.cells} cells, {s.mlups:.0f} MLUPS, "
+         f"{s.seconds:.1f} s")
+    assert abs(cd / refc - 1.0) < 0.03
+    assert np.isfinite(r.surface_pressure_pa).sum() > 0.9 * r.surface_pressure_pa.size
+    sl = r.slices[0]
+    assert np.isfinite(sl.fields[0].data).sum() > sl.fields[0].data.size / 2
+    cfg, m = T.sphere_case(100.0, 48)
+    p = T.plan(cfg, m, 1e-3, T.SolverOptions())
+    assert p.multi is not None and s.cells == sum(p.multi.cells) and s.grid == p.dims
+
+
+@pytest.mark.parametrize("re,d,passes,tol,refine", [(100.0, 48, 30.0, 0.03, True), (300.0, 48, 40.0, 0.06, True),
+                                                     (100.0, 48, 30.0, 0.03, False)])
+def test_sphere_on_48_cells(re, d, passes, tol, refine):
     # tunnel.rs tests: sphere_re100 and sphere_re300 (sphere_check with the
-    # reference's bounds). The reference runs these on its refined D3Q27
-    # grid; here the same plan on one uniform D3Q19 grid (46.8 M cells).
-    _say(f"study: sphere Re {re:.0f}, {d} cells across, {passes:.0f} passes")
-    cd, r = T.sphere_drag(re, d, passes)
+    # reference's bounds), on the refined D3Q27 grid as the reference plans
+    # them, and on one uniform D3Q19 grid (refine = false) at Re 100
+    _say(f"study: sphere Re {re:.0f}, {d} cells across, {passes:.0f} passes, refine {refine}")
+    cd, r = T.sphere_drag(re, d, passes, T.SolverOptions(refine=refine))
     s = r.summary
     refc = T.sphere_cd_cgw(re) if re > 250.0 else 0.5 * (T.sphere_cd_cgw(re) + T.sphere_cd_sn(re))
     _say(f"  Cd {cd:.4f} vs {refc:.4f} ({(cd / refc - 1) * 100:+.2f} %), converged {s.converged}, "

@@ -148,26 +148,72 @@ def test_memory_budget_selects_fp16():
     assert tight.storage == Storage.FP16 and tight.memory < big.memory
 
 
-def test_slender_part_sizing_on_a_uniform_grid():
-    # tunnel.rs tests: slender_parts_get_cells_across_their_thickness. A
-    # sphere keeps the quality's grid; a custom grid is kept with a warning;
-    # a slender arm asks for more cells across (cells_for_thin) but on one
-    # uniform grid the time cap holds it at the quality's, so the estimate
-    # says its thickness is under-resolved (the reference reaches the target
-    # through its refined grid, which this build does not have).
+def test_slender_parts_get_cells_across_their_thickness():
+    # tunnel.rs tests: slender_parts_get_cells_across_their_thickness (tests/tunnel.rs:146-174)
+    arm = cuboid([-0.003, -0.075, -0.006], [0.003, 0.075, 0.006])
+    for q in ("draft", "normal"):
+        e = T.estimate(T.WindConfig(speed_m_s=15.0, quality=q), arm, 1e-3, 32_000_000_000)
+        assert abs(e.thin_mm - 12.0) < 0.01
+        assert e.cells_thin >= T.thin_cells_target(q) - 0.5, (q, e.cells_thin)
+        assert e.thin_warning is None
+        assert "grid levels" in e.note
+    # a compact part keeps the quality's grid
     cfg = T.WindConfig(speed_m_s=15.0, quality="draft")
     ball = sphere([0.0, 0.0, 0.0], 0.01, 4)
     assert T.estimate(cfg, ball, 1e-3, 32_000_000_000).cells_across == 64
-    arm = cuboid([-0.003, -0.075, -0.006], [0.003, 0.075, 0.006])
+    # a small memory budget caps the refinement, and the estimate says so
+    e = T.estimate(T.WindConfig(speed_m_s=15.0, quality="normal"), arm, 1e-3, 600_000_000)
+    assert e.cells_thin < 24.0 and e.thin_warning is not None and e.memory_bytes <= 600_000_000
+    # a custom grid is kept as asked, with the warning
     e = T.estimate(T.WindConfig(speed_m_s=15.0, quality=64), arm, 1e-3, 32_000_000_000)
     assert e.cells_across == 64 and e.thin_warning is not None
-    assert abs(e.thin_mm - 12.0) < 0.01
+    assert T.cells_for_thin(64, 16, 0.15, 0.012) == 200
+    assert T.cells_for_thin(64, 16, 0.02, 0.02) == 64
+    assert T.cells_for_thin(64, 16, 1.0, 0.001) == 64 * T.THIN_MAX_BOOST
+
+
+def test_slender_part_on_one_uniform_grid():
+    # refine = false: one uniform D3Q19 grid, where the time cap holds the
+    # arm near the quality's grid and the estimate says its thickness is
+    # under-resolved
+    opt = T.SolverOptions(refine=False)
+    arm = cuboid([-0.003, -0.075, -0.006], [0.003, 0.075, 0.006])
     for q, target, want in (("draft", 16, 200), ("normal", 24, 300)):
-        e = T.estimate(T.WindConfig(speed_m_s=15.0, quality=q), arm, 1e-3, 32_000_000_000)
+        e = T.estimate(T.WindConfig(speed_m_s=15.0, quality=q), arm, 1e-3, 32_000_000_000, opt)
         assert abs(e.thin_mm - 12.0) < 0.01
-        assert e.cells_across >= T.cells_across(q)
+        assert e.cells_across >= T.cells_across(q) and "grid levels" not in e.note
         if e.cells_thin < target - 0.5:
             assert e.thin_warning is not None and f"{want} cells" in e.thin_warning
+
+
+def test_refined_plan_nests_boxes_round_the_part():
+    """tunnel.rs:467-475 and 643-687: levels from the cells across (the
+    coarsest keeps 24 or more), coarse dims that make the finest spacing
+    dx, each box inside its parent with the solver's margins, cell updates
+    per finest step summed over levels."""
+    assert [T.still_levels(n) for n in (16, 47, 48, 95, 96, 192, 400)] == [1, 1, 2, 2, 3, 4, 4]
+    cfg, m = T.sphere_case(100.0, 48)
+    p = T.plan(cfg, m, 1e-3, T.SolverOptions())
+    mp = p.multi
+    assert mp is not None and len(mp.cells) == 2
+    assert all(p.dims[k] == (mp.dims0[k] - 1) * 2 + 1 for k in range(3))
+    dims = [tuple(mp.dims0)]
+    for l, b in enumerate(mp.boxes, start=1):
+        margin = 2 if l == 1 else 3
+        for k in range(3):
+            assert b.lo[k] >= margin and b.lo[k] + b.n[k] - 1 <= dims[-1][k] - 1 - margin and b.n[k] >= 8
+        dims.append(tuple(2 * (b.n[k] - 1) + 1 for k in range(3)))
+    assert mp.cells == [int(np.prod(d)) for d in dims]
+    assert p.updates_per_step == pytest.approx(sum(c * 2 ** l for l, c in enumerate(mp.cells)) / 2)
+    # the part lies inside the finest box: its centre, in finest cells from org_f
+    c = [p.center[k] - mp.org_f[k] for k in range(3)]
+    r = 0.025 / p.dx
+    assert all(r + 2 < c[k] < dims[-1][k] - r - 2 for k in range(3))
+    # refine = false plans the uniform grid
+    assert T.plan(cfg, m, 1e-3, T.SolverOptions(refine=False)).multi is None
+    # spinning parts never refine
+    spin = T.WindConfig(wind_dir=(0, 0, 1), speed_m_s=0.0, spin=T.Spin(20000.0, (0, 0, 1)), quality=64)
+    assert T.plan(spin, PropSpec.five_inch().mesh(), 1e-3, T.SolverOptions()).multi is None
 
 
 def test_result_files_round_trip(tmp_path):
