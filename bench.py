@@ -245,6 +245,47 @@ def study_batch(sa, W, steps, sizes=(128, 192, 256, 384)):
     return out
 
 
+def study_refined(sa, steps):
+    """The refined D3Q27 solver on the study driver's plan of a sphere at
+    Re 100 with 48 and 96 cells across (two and three levels, sponges,
+    total-pressure inlet): cell updates per second over all levels (each
+    level's cells times its steps), FP32 and FP16."""
+    from simlab_amd import tunnel as T
+    from simlab_amd import mlattice as ml
+    import numpy as np
+    out = {}
+    for across in (48, 96):
+        cfg, m = T.sphere_case(100.0, across)
+        p = T.plan(cfg, m, 1e-3, T.SolverOptions())
+        mp = p.multi
+        P = np.asarray(m.positions, np.float64)
+        to_lat = np.asarray(p.center) + ((P - p.pref) @ p.r.T) / p.dx - np.asarray(mp.org_f)
+        b = mp.boxes[-1]
+        fd = tuple(2 * (int(b.n[k]) - 1) + 1 for k in range(3))
+        vox = ml.voxelize_static27(to_lat, m.triangles, fd)
+        nl = len(mp.cells)
+        for name, st in (("fp32", sa.Storage.FP32), ("fp16", sa.Storage.FP16)):
+            s = ml.MultiSetup(dims0=mp.dims0, boxes=list(mp.boxes), faces=p.faces, nu0=p.nu_lat / float(1 << (nl - 1)),
+                              u_inf=(p.u_lat, 0, 0), u_init=(p.u_lat, 0, 0), sponge=p.sponge, sponge_tau=1.0,
+                              sponge_sigma=0.1, total_pressure_inlet=True, storage=st,
+                              center=tuple(p.center[k] - mp.org_f[k] for k in range(3)))
+            lat = ml.MultiLattice(s, sa.Geometry.static(vox))
+            ss = lat.sub_steps()
+            lat.run_still(20 * ss)
+            n = max(1, steps // ss) * ss
+            t0 = time.perf_counter()
+            lat.run_still(n)
+            wall = time.perf_counter() - t0
+            upd = sum(c * float(1 << l) for l, c in enumerate(mp.cells)) / float(ss) * n
+            out[f"sphere{across}_{name}"] = {"levels": nl, "level_cells": mp.cells, "nrec": lat.nrec,
+                                             "fine_steps": n, "mlups": round(upd / wall / 1e6, 1),
+                                             "ms_per_coarse_step": round(wall / (n / ss) * 1e3, 4)}
+            print(f"# refined sphere {across} {name}: {out[f'sphere{across}_{name}']['mlups']} MLUPS",
+                  file=sys.stderr, flush=True)
+            lat.close()
+    return out
+
+
 def drag_run(sa, W, storage, steps):
     setup, geom, meta = W.sphere_channel_large(storage)
     lat = sa.Lattice(setup, geom)
@@ -312,6 +353,27 @@ def traffic_from_profile(storage_name):
     return d.get("bytes_per_launch")
 
 
+def pattern_probe(reps):
+    """Memory ceilings of the streaming access patterns at 384^3 FP16 (no
+    arithmetic): AA even and odd steps in place, a two-lattice pull step and
+    an aligned out-of-place step (simlab_cfd_probe_stream_patterns), beside
+    a plain device copy (copy_calibration's kernel)."""
+    import ctypes as C
+    import simlab_amd as sa
+    lib = sa.load_library()
+    lib.simlab_cfd_probe_stream_patterns.restype = C.c_int
+    lib.simlab_cfd_probe_stream_patterns.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    g = (C.c_double * 4)()
+    rc = lib.simlab_cfd_probe_stream_patterns(384, 384, 384, max(int(reps), 1), g)
+    if rc != 0:
+        raise RuntimeError(lib.simlab_cfd_last_error().decode())
+    out = {"grid": [384, 384, 384], "bytes_per_cell": 76, "unit": "GB/s",
+           "aa_even_in_place": round(g[0], 1), "aa_odd_in_place": round(g[1], 1),
+           "aa_mean": round(2.0 / (1.0 / g[0] + 1.0 / g[1]), 1),
+           "two_lattice_pull": round(g[2], 1), "aligned_out_of_place": round(g[3], 1)}
+    return out
+
+
 def copy_calibration(reps):
     """A kernel of known traffic: out = x * 1 over 2^28 floats reads and writes
     exactly 1 GiB each per launch."""
@@ -343,10 +405,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--no-384", action="store_true", help="skip the 384^3 sphere-channel and spinning-prop timings")
-    ap.add_argument("--study", choices=["prop", "arm", "batch"], default=None,
+    ap.add_argument("--study", choices=["prop", "arm", "batch", "refined"], default=None,
                     help="accuracy studies instead of the benchmark line: the 384^3 spinning prop's thrust and "
                          "torque per storage (prop), or the 512x256x256 drone arm's drag per storage plus the "
-                         "batched arm at four sizes (arm); one JSON object")
+                         "batched arm at four sizes (arm), the batch timings alone (batch), or the refined D3Q27 "
+                         "solver's MLUPS on the study plans of a sphere at 48 and 96 cells across (refined); "
+                         "one JSON object")
     ap.add_argument("--study-steps", type=int, default=0, help="steps of a study run (prop 20000, arm 10000)")
     ap.add_argument("--only-384", action="store_true",
                     help="only the 384^3 timings (one JSON object of them; for profiling those kernels alone)")
@@ -354,6 +418,9 @@ def main():
                                                   "fp16,fp32")
     ap.add_argument("--case", default="sphere,prop", help="with --only-384: comma list of the 384^3 cases to time "
                                                          "(sphere, prop)")
+    ap.add_argument("--pattern-probe", action="store_true",
+                    help="only measure the memory ceilings of the streaming access patterns at 384^3 FP16 (AA even "
+                         "and odd steps in place, two-lattice pull, aligned out of place), with no arithmetic")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profile into bytes)")
@@ -366,6 +433,9 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.steps)
         return
+    if args.pattern_probe:
+        print(json.dumps({"pattern_probe": pattern_probe(min(args.steps, 50))}), flush=True)
+        return
 
     import simlab_amd as sa
     from simlab_amd import workloads as W
@@ -374,6 +444,9 @@ def main():
     print(f"# {info}", file=sys.stderr, flush=True)
     if args.study == "prop":
         print(json.dumps({"study_prop384": study_prop(sa, W, args.study_steps or 20000)}), flush=True)
+        return
+    if args.study == "refined":
+        print(json.dumps({"study_refined": study_refined(sa, args.study_steps or 400)}), flush=True)
         return
     if args.study == "batch":
         print(json.dumps({"study_batch": study_batch(sa, W, args.study_steps or 300)}), flush=True)

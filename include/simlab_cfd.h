@@ -1,11 +1,14 @@
 /*
- * simlab_cfd.h — C ABI of the MI355X (gfx950) D3Q19 wind-tunnel lattice.
+ * simlab_cfd.h — C ABI of the MI355X (gfx950) D3Q19 wind-tunnel lattice and
+ * the refined D3Q27 solver.
  *
  * Drop-in boundary for the uniform-grid lattice of SimLab's `crates/cfd`
- * (crates/cfd/src/lattice.rs:220-841). The Rust `Lattice` type forwards each
- * of its methods to one function here; the Rust side of that binding is shown
- * in INTEGRATION.md. Everything is in lattice units (cell size 1, time step
- * 1, reference density 1), as in the reference.
+ * (crates/cfd/src/lattice.rs:220-841) and its refined multi-level lattice
+ * (crates/cfd/src/mlattice.rs:139-948). The Rust `Lattice` and
+ * `MultiLattice` types forward each of their methods to one function here;
+ * the Rust side of that binding is shown in INTEGRATION.md. Everything is in
+ * lattice units (cell size 1, time step 1, reference density 1), as in the
+ * reference.
  *
  * Conventions
  *  - Every function returns SIMLAB_OK (0) or an error code; the message of
@@ -448,6 +451,15 @@ int simlab_cfd_multi_slice_means(simlab_multi *m, float *out);
  * deviations f - w as doubles, direction i of cell n at i * cells + n. For
  * tests. out: 27 x cells(l) doubles. */
 int simlab_cfd_multi_get_pops(simlab_multi *m, uint32_t level, double *out);
+
+/* Measurement only (no counterpart in the reference): the HBM bandwidth the
+ * streaming access patterns of one D3Q19 step move with no arithmetic, on
+ * the grouped 16-bit layout of a periodic nx x ny x nz grid (cells a multiple
+ * of 64): gbs[0] the AA even step in place, gbs[1] the AA odd step in place,
+ * gbs[2] a two-lattice pull step, gbs[3] an aligned out-of-place step; GB/s
+ * of the 76 B per cell each moves, averaged over reps passes. Allocates two
+ * population buffers for the call. */
+int simlab_cfd_probe_stream_patterns(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t reps, double gbs[4]);
 
 /* `inside_grid` (voxel.rs:73-95): inside flags (0/1) for the grid points
  * origin + h * (i, j, k), i fastest. out: dims[0]*dims[1]*dims[2] bytes. */

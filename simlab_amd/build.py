@@ -60,6 +60,8 @@ def build(verbose=False, save_temps=False):
     jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "mkernels.hip"), "-o", obj], obj))
     obj = os.path.join(OUT, "mlattice.cpp.o")
     jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "mlattice.cpp"), "-o", obj], obj))
+    obj = os.path.join(OUT, "probe.o")
+    jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "probe.hip"), "-o", obj], obj))
     obj = os.path.join(OUT, "voxel_gpu.o")
     jobs.append(([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, "voxel_gpu.hip"), "-o", obj], obj))
     obj = os.path.join(OUT, "lattice.cpp.o")
