@@ -55,6 +55,11 @@ struct DevLattice {
     // 1: bulk blocks are taken in XCD order (the GPU deals workgroups to its
     // 8 XCDs in turn; the swizzle gives each XCD a contiguous run of tiles)
     uint32_t xcd_swizzle;
+    // 1: rows start on a group boundary (nx a multiple of POP_GROUP) and the
+    // 16-bit population buffer holds under 2^31 elements, so the bulk
+    // kernel's parity-1 step addresses every neighbour slot as one of three
+    // per-lane offsets plus a wave-uniform shift
+    uint32_t rows_aligned;
 
     uint32_t nrec;
     uint32_t *rmask;             // [nrec]

@@ -248,6 +248,64 @@ template <class C> __device__ __forceinline__ void scatter_pre0(const DevLattice
     });
 }
 
+// ---------------------------------------------------------------- parity 1 on aligned rows
+// With nx a multiple of POP_GROUP every row starts a group, so the slot of
+// cell (x + dx, y + dy, z + dz) is element ex(dx) + sy(dy) + sz(dz) + i * 64:
+// ex one of three per-lane values (the cell and its x-neighbours, wrapped
+// within the row), sy and sz wave-uniform multiples of a group (a wave's 64
+// cells share y and z). The parity-1 step's 19 loads and 19 stores then
+// take three offset registers and scalar bases instead of a 64-bit address
+// per access. The slots, values and dither are those of gather_pre<1> and
+// scatter_post<1, false>.
+struct RowAddr {
+    uint32_t e[3];           // byte offset of slot 0 of x - 1, x, x + 1 (wrapped within the row) from the row's base
+    int32_t sy[3], sz[3];    // element shift of a row / plane step of -1, 0, +1 (wrapped); wave-uniform
+    __amdgpu_buffer_rsrc_t r;   // the population buffer from the lowest row / plane the wave touches
+    int32_t lo;              // that row's element shift
+};
+__device__ __forceinline__ uint32_t slot_elem(uint32_t n) { return (n / POP_GROUP) * (19 * POP_GROUP) + (n % POP_GROUP); }
+template <class C> __device__ __forceinline__ RowAddr row_addr(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
+    static_assert(sizeof(typename C::E) == 2, "16-bit storage");
+    RowAddr r;
+    r.e[0] = 2u * slot_elem(x == 0 ? n + L.nx - 1 : n - 1);
+    r.e[1] = 2u * slot_elem(n);
+    r.e[2] = 2u * slot_elem(x == L.nx - 1 ? n + 1 - L.nx : n + 1);
+    const int32_t gr = (int32_t)(L.nx / POP_GROUP) * (19 * (int32_t)POP_GROUP);   // one row
+    const int32_t gp = gr * (int32_t)L.ny;                                          // one plane
+    r.sy[0] = y == 0 ? gr * (int32_t)(L.ny - 1) : -gr;
+    r.sy[1] = 0;
+    r.sy[2] = y == L.ny - 1 ? -gr * (int32_t)(L.ny - 1) : gr;
+    r.sz[0] = z == 0 ? gp * (int32_t)(L.nz - 1) : -gp;
+    r.sz[1] = 0;
+    r.sz[2] = z == L.nz - 1 ? -gp * (int32_t)(L.nz - 1) : gp;
+    r.lo = min(min(r.sy[0], r.sy[2]), 0) + min(min(r.sz[0], r.sz[2]), 0);
+    // GCN/CDNA raw buffer: 0x00020000 in the last word (32-bit data format),
+    // no range limit; every scalar offset below is >= 0 from this base
+    char *base = (char *)L.pops + 2 * (int64_t)r.lo;
+    r.r = __builtin_amdgcn_make_buffer_rsrc(base, 0, -1, 0x00020000);
+    return r;
+}
+// input i of a parity-1 step: slot OPP[i] of cell n - c_i
+template <class C> __device__ __forceinline__ void gather_pre_rows(const RowAddr &r, typename C::T f[19]) {
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        const int32_t so = 2 * (r.sy[1 - CY[i]] + r.sz[1 - CZ[i]] - r.lo + OPP[i] * (int32_t)POP_GROUP);
+        const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r.r, (int32_t)r.e[1 - CX[i]], so, 0);
+        f[i] = dec<C>(h);
+    });
+}
+// output i of a parity-1 step: slot i of cell n + c_i (slot 0 of n itself)
+template <class C>
+__device__ __forceinline__ void scatter_post_rows(const RowAddr &r, uint32_t n, const typename C::T f[19], uint32_t seed) {
+    Dither d;
+    d.init(n, seed);
+    sfor<19>([&](auto I) {
+        constexpr int i = I;
+        const int32_t so = 2 * (r.sy[1 + CY[i]] + r.sz[1 + CZ[i]] - r.lo + i * (int32_t)POP_GROUP);
+        __builtin_amdgcn_raw_buffer_store_b16(enc<C>(f[i], d.template d16<i>()), r.r, (int32_t)r.e[1 + CX[i]], so, 0);
+    });
+}
+
 // ---------------------------------------------------------------- moments
 // shader.rs:341-365, summed over +-pairs (a = f_i + f_opp, b = f_i - f_opp)
 template <class T> struct Mom {
@@ -1087,9 +1145,10 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t nb) {
 // stay inside the block and waves touch whole cache lines.
 // tile bxy (of xblocks * yblocks) of z-plane z; thread tid of the tile's
 // bx * by threads (x fastest); threads past the tile idle
-template <class C, int P, bool FORCE, bool SPONGE>
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
 __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &a, uint32_t bxy, uint32_t z, uint32_t tid) {
     using T = typename C::T;
+    static_assert(!ROWS || P == 1, "row-aligned addressing is for the parity-1 step");
     // A wave is 64 consecutive x of one row (bx is a multiple of 64, tiles
     // are whole waves): the tile, the row and the plane are the same for all
     // its lanes, so they (and the row and plane neighbour offsets) live in
@@ -1121,8 +1180,13 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     const bool may = inside && !is_face_cell(L, x, y, z);
     Nbr nb;
     nb.make(L, x, y, z, n);
+    RowAddr ra;
+    if constexpr (ROWS) ra = row_addr<C>(L, x, y, z, n);
     T f[19];
-    if (may) gather_pre<C, P>(L, nb, f);
+    if (may) {
+        if constexpr (ROWS) gather_pre_rows<C>(ra, f);
+        else gather_pre<C, P>(L, nb, f);
+    }
     // ours: fluid cells without a record (plain or halfway wall cells);
     // solid, face and wall-record cells are not
     const bool mine = may && (fg & 3u) == 0 && rec_of(fg) == 0;
@@ -1131,7 +1195,8 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     if (mine) {
         if (hwm) halfway_links<C, P>(L, a, nb, hwm, f);
         collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
-        scatter_post<C, P, false>(L, nb, f, a.seed);
+        if constexpr (ROWS) scatter_post_rows<C>(ra, n, f, a.seed);
+        else scatter_post<C, P, false>(L, nb, f, a.seed);
     }
     const bool src = mine && is_face_source(L, x, y, z);
     if (__ballot(src)) {
@@ -1198,7 +1263,7 @@ template <class C, int P, bool FORCE, bool SPONGE> constexpr int bulk_waves() {
 struct ZMap {
     uint32_t zbase, zskip_at, zskip_len;
 };
-template <class C, int P, bool FORCE, bool SPONGE>
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(bulk_waves<C, P, FORCE, SPONGE>())))
 k_bulk(DevLattice L, StepArgs a, ZMap zm) {
     uint32_t bxy = blockIdx.x, zl = blockIdx.y;
@@ -1209,7 +1274,7 @@ k_bulk(DevLattice L, StepArgs a, ZMap zm) {
     }
     uint32_t z = zm.zbase + zl;
     if (z >= zm.zskip_at) z += zm.zskip_len;
-    bulk_block<C, P, FORCE, SPONGE>(L, a, bxy, z, threadIdx.y * blockDim.x + threadIdx.x);
+    bulk_block<C, P, FORCE, SPONGE, ROWS>(L, a, bxy, z, threadIdx.y * blockDim.x + threadIdx.x);
 }
 
 // One launch per step, one stream: blocks [0, wall_blocks + hw_blocks) do
@@ -1224,7 +1289,7 @@ k_bulk(DevLattice L, StepArgs a, ZMap zm) {
 template <class C, int P, bool FORCE, bool SPONGE> constexpr int step_waves() {
     return ((std::is_same_v<C, CF16> || std::is_same_v<C, CF32>) && P == 1 && !FORCE && !SPONGE) ? 5 : 1;
 }
-template <class C, int P, bool FORCE, bool SPONGE>
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
 __global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE>())))
 k_step(DevLattice L, StepArgs a) {
     __shared__ double sh[16][6];
@@ -1239,7 +1304,7 @@ k_step(DevLattice L, StepArgs a) {
     const uint32_t g = b * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;
     if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
-    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
+    bulk_block<C, P, FORCE, SPONGE, ROWS>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 // ---------------------------------------------------------------- batches
@@ -2170,12 +2235,21 @@ hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent
     return hipGetLastError();
 }
 
+// the row-aligned parity-1 variant where the lattice allows it (16-bit storage)
+template <class C, int P> constexpr bool rows_variant() {
+    return P == 1 && (std::is_same_v<C, CF16> || std::is_same_v<C, CBF16>);
+}
+
 template <class C, int P, bool FORCE, bool SPONGE>
 hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk, hipEvent_t ev0,
                         hipEvent_t ev1) {
     k_special<C, P, FORCE, SPONGE><<<L.wall_blocks + L.hw_blocks, SPECIAL_BLOCK, 0, s_special>>>(L, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if constexpr (rows_variant<C, P>())
+        if (L.rows_aligned)
+            return launch_timed(k_bulk<C, P, FORCE, SPONGE, true>, dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by),
+                                s_bulk, ev0, ev1, L, a, ZMap{0u, L.nz, 0u});
     return launch_timed(k_bulk<C, P, FORCE, SPONGE>, dim3(L.xblocks * L.yblocks, L.nz), dim3(L.bx, L.by), s_bulk, ev0, ev1,
                         L, a, ZMap{0u, L.nz, 0u});
 }
@@ -2190,6 +2264,10 @@ template <class C, int P, bool FORCE, bool SPONGE>
 hipError_t bulk_z_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s, uint32_t nplanes, const ZMap &zm,
                           hipEvent_t ev0, hipEvent_t ev1) {
     if (nplanes == 0) return hipSuccess;
+    if constexpr (rows_variant<C, P>())
+        if (L.rows_aligned)
+            return launch_timed(k_bulk<C, P, FORCE, SPONGE, true>, dim3(L.xblocks * L.yblocks, nplanes), dim3(L.bx, L.by), s,
+                                ev0, ev1, L, a, zm);
     return launch_timed(k_bulk<C, P, FORCE, SPONGE>, dim3(L.xblocks * L.yblocks, nplanes), dim3(L.bx, L.by), s, ev0, ev1,
                         L, a, zm);
 }
@@ -2209,6 +2287,8 @@ hipError_t fused_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s, 
     const uint64_t tiles = (uint64_t)L.xblocks * L.yblocks * L.nz;
     const uint32_t per = SPECIAL_BLOCK / (L.bx * L.by);
     const uint32_t nb = L.wall_blocks + L.hw_blocks + cdiv(tiles, per);
+    if constexpr (rows_variant<C, P>())
+        if (L.rows_aligned) return launch_timed(k_step<C, P, FORCE, SPONGE, true>, dim3(nb), dim3(SPECIAL_BLOCK), s, ev0, ev1, L, a);
     return launch_timed(k_step<C, P, FORCE, SPONGE>, dim3(nb), dim3(SPECIAL_BLOCK), s, ev0, ev1, L, a);
 }
 

@@ -551,6 +551,10 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         lat->L.plain = (e && e[0] == '0') ? nullptr : lat->plain.as<uint32_t>();
         const char *sw = std::getenv("SIMLAB_CFD_SWIZZLE");
         lat->L.xcd_swizzle = (sw && sw[0] == '0') ? 0u : 1u;
+        const char *ra = std::getenv("SIMLAB_CFD_ROWS");
+        const bool half = lat->storage == SIMLAB_STORAGE_FP16 || lat->storage == SIMLAB_STORAGE_BF16;
+        lat->L.rows_aligned =
+            (!(ra && ra[0] == '0') && half && lat->setup.dims[0] % POP_GROUP == 0 && pop_elems(n) < (1ull << 31)) ? 1u : 0u;
     }
     up(lat->rmask, h_mask.data(), (size_t)nrec * 4);
     up(lat->rcell, h_cell.data(), (size_t)nrec * 4);

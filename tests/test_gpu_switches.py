@@ -62,3 +62,25 @@ def test_plain_groups_and_xcd_order_change_nothing(monkeypatch, launch):
         lat = sa.Lattice(setup, geom)
         out[plain] = (lat.run_still(25), lat.get_pops())
     assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
+@pytest.mark.parametrize("launch,sponge", [("fused", False), ("split", False), ("split", True)])
+def test_row_aligned_addressing_changes_nothing(monkeypatch, storage, launch, sponge):
+    # rows of a multiple of 64 cells: the parity-1 step addresses its slots
+    # from three per-lane offsets and scalar shifts (buffer loads and
+    # stores); the same slots, values and dither as the generic addressing,
+    # so bit-identical populations and forces (the rows wrap at y, z = 0 and
+    # the far edge, the periodic sides of the channel)
+    print(f"case rows on/off {storage.name} {launch} sponge={sponge}", flush=True)
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", launch)
+    out = {}
+    for rows in ("1", "0"):
+        monkeypatch.setenv("SIMLAB_CFD_ROWS", rows)
+        setup, geom, meta = W.sphere_channel((128, 64, 48), 6.0, 30.0, 200.0, 0.05, 0.1, storage, subdiv=3)
+        if sponge:
+            setup.sponge = (0, 12, 4, 4, 4, 4)
+            setup.sponge_tau, setup.sponge_sigma = 0.9, 0.05
+        lat = sa.Lattice(setup, geom)
+        out[rows] = (lat.run_still(25), lat.get_pops())
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
