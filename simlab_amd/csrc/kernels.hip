@@ -250,59 +250,76 @@ template <class C> __device__ __forceinline__ void scatter_pre0(const DevLattice
 
 // ---------------------------------------------------------------- parity 1 on aligned rows
 // With nx a multiple of POP_GROUP every row starts a group, so the slot of
-// cell (x + dx, y + dy, z + dz) is element ex(dx) + sy(dy) + sz(dz) + i * 64:
-// ex one of three per-lane values (the cell and its x-neighbours, wrapped
-// within the row), sy and sz wave-uniform multiples of a group (a wave's 64
-// cells share y and z). The parity-1 step's 19 loads and 19 stores then
-// take three offset registers and scalar bases instead of a 64-bit address
-// per access. The slots, values and dither are those of gather_pre<1> and
-// scatter_post<1, false>.
+// cell (x + dx, y + dy, z + dz) is element plane(z + dz) + row(y) + ex(dx) +
+// sy(dy) + i * 64: ex one of three per-lane values (the cell and its
+// x-neighbours, wrapped within the row), the plane, row and sy wave-uniform
+// (a wave's 64 cells share y and z). The parity-1 step's 19 loads and 19
+// stores then take three offset registers, one buffer base per z-plane and
+// scalar offsets instead of a 64-bit address per access. The slots, values
+// and dither are those of gather_pre<1> and scatter_post<1, false>.
 struct RowAddr {
-    uint32_t e[3];           // byte offset of slot 0 of x - 1, x, x + 1 (wrapped within the row) from the row's base
-    int32_t sy[3], sz[3];    // element shift of a row / plane step of -1, 0, +1 (wrapped); wave-uniform
-    __amdgpu_buffer_rsrc_t r;   // the population buffer from the lowest row / plane the wave touches
-    int32_t lo;              // that row's element shift
+    uint32_t e[3];                // byte offset of slot 0 of x - 1, x, x + 1 (wrapped within the row) from the row's start
+    int32_t sy[3];                // element shift of a row step of -1, 0, +1 (wrapped) less lo; wave-uniform, >= 0
+    __amdgpu_buffer_rsrc_t r[3];  // the buffer from the lowest row the wave touches in the plane z - 1, z, z + 1 (wrapped)
 };
-__device__ __forceinline__ uint32_t slot_elem(uint32_t n) { return (n / POP_GROUP) * (19 * POP_GROUP) + (n % POP_GROUP); }
-template <class C> __device__ __forceinline__ RowAddr row_addr(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
-    static_assert(sizeof(typename C::E) == 2, "16-bit storage");
+// element of slot 0 of the cell x of a row, from the row's first slot
+__device__ __forceinline__ uint32_t row_elem(uint32_t x) { return (x / POP_GROUP) * (19 * POP_GROUP) + (x % POP_GROUP); }
+template <class C> __device__ __forceinline__ RowAddr row_addr(const DevLattice &L, uint32_t x, uint32_t yl, uint32_t zl) {
+    constexpr uint32_t ES = sizeof(typename C::E);
+    static_assert(ES == 2 || ES == 4, "16- or 32-bit storage");
+    // every lane of the wave holds the same y and z (whole wave active here)
+    const uint32_t y = __builtin_amdgcn_readfirstlane(yl), z = __builtin_amdgcn_readfirstlane(zl);
     RowAddr r;
-    r.e[0] = 2u * slot_elem(x == 0 ? n + L.nx - 1 : n - 1);
-    r.e[1] = 2u * slot_elem(n);
-    r.e[2] = 2u * slot_elem(x == L.nx - 1 ? n + 1 - L.nx : n + 1);
+    r.e[0] = ES * row_elem(x == 0 ? L.nx - 1 : x - 1);
+    r.e[1] = ES * row_elem(x);
+    r.e[2] = ES * row_elem(x == L.nx - 1 ? 0 : x + 1);
     const int32_t gr = (int32_t)(L.nx / POP_GROUP) * (19 * (int32_t)POP_GROUP);   // one row
-    const int32_t gp = gr * (int32_t)L.ny;                                          // one plane
-    r.sy[0] = y == 0 ? gr * (int32_t)(L.ny - 1) : -gr;
-    r.sy[1] = 0;
-    r.sy[2] = y == L.ny - 1 ? -gr * (int32_t)(L.ny - 1) : gr;
-    r.sz[0] = z == 0 ? gp * (int32_t)(L.nz - 1) : -gp;
-    r.sz[1] = 0;
-    r.sz[2] = z == L.nz - 1 ? -gp * (int32_t)(L.nz - 1) : gp;
-    r.lo = min(min(r.sy[0], r.sy[2]), 0) + min(min(r.sz[0], r.sz[2]), 0);
+    const int32_t s0 = y == 0 ? gr * (int32_t)(L.ny - 1) : -gr, s2 = y == L.ny - 1 ? -gr * (int32_t)(L.ny - 1) : gr;
+    const int32_t lo = min(min(s0, s2), 0);
+    // (held in scalar registers: they are the accesses' scalar offsets)
+    r.sy[0] = __builtin_amdgcn_readfirstlane(s0 - lo);
+    r.sy[1] = __builtin_amdgcn_readfirstlane(-lo);
+    r.sy[2] = __builtin_amdgcn_readfirstlane(s2 - lo);
     // GCN/CDNA raw buffer: 0x00020000 in the last word (32-bit data format),
-    // no range limit; every scalar offset below is >= 0 from this base
-    char *base = (char *)L.pops + 2 * (int64_t)r.lo;
-    r.r = __builtin_amdgcn_make_buffer_rsrc(base, 0, -1, 0x00020000);
+    // no range limit; every offset from these bases is >= 0 and within one
+    // plane (below 2^31 bytes, checked when the lattice is built)
+    const uint32_t zs[3] = {z == 0 ? L.nz - 1 : z - 1, z, z == L.nz - 1 ? 0 : z + 1};
+    for (int k = 0; k < 3; ++k) {
+        const int64_t row0 = ((int64_t)zs[k] * L.ny + y) * gr + lo;
+        r.r[k] = __builtin_amdgcn_make_buffer_rsrc((char *)L.pops + (int64_t)ES * row0, 0, -1, 0x00020000);
+    }
     return r;
+}
+template <class C>
+__device__ __forceinline__ typename C::E row_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, int32_t so) {
+    if constexpr (sizeof(typename C::E) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, (int32_t)vo, so, 0);
+    else return __builtin_bit_cast(typename C::E, __builtin_amdgcn_raw_buffer_load_b32(r, (int32_t)vo, so, 0));
+}
+template <class C>
+__device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t r, typename C::E v, uint32_t vo, int32_t so) {
+    if constexpr (sizeof(typename C::E) == 2) __builtin_amdgcn_raw_buffer_store_b16(v, r, (int32_t)vo, so, 0);
+    else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int32_t)vo, so, 0);
 }
 // input i of a parity-1 step: slot OPP[i] of cell n - c_i
 template <class C> __device__ __forceinline__ void gather_pre_rows(const RowAddr &r, typename C::T f[19]) {
+    constexpr uint32_t ES = sizeof(typename C::E);
     sfor<19>([&](auto I) {
         constexpr int i = I;
-        const int32_t so = 2 * (r.sy[1 - CY[i]] + r.sz[1 - CZ[i]] - r.lo + OPP[i] * (int32_t)POP_GROUP);
-        const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r.r, (int32_t)r.e[1 - CX[i]], so, 0);
-        f[i] = dec<C>(h);
+        // the slot's constant offset goes with the lane offset, where it
+        // folds into the instruction's immediate offset
+        f[i] = dec<C>(row_load<C>(r.r[1 - CZ[i]], r.e[1 - CX[i]] + ES * OPP[i] * POP_GROUP, ES * r.sy[1 - CY[i]]));
     });
 }
 // output i of a parity-1 step: slot i of cell n + c_i (slot 0 of n itself)
 template <class C>
 __device__ __forceinline__ void scatter_post_rows(const RowAddr &r, uint32_t n, const typename C::T f[19], uint32_t seed) {
+    constexpr uint32_t ES = sizeof(typename C::E);
     Dither d;
-    d.init(n, seed);
+    if constexpr (C::SR) d.init(n, seed);
     sfor<19>([&](auto I) {
         constexpr int i = I;
-        const int32_t so = 2 * (r.sy[1 + CY[i]] + r.sz[1 + CZ[i]] - r.lo + i * (int32_t)POP_GROUP);
-        __builtin_amdgcn_raw_buffer_store_b16(enc<C>(f[i], d.template d16<i>()), r.r, (int32_t)r.e[1 + CX[i]], so, 0);
+        row_store<C>(r.r[1 + CZ[i]], enc<C>(f[i], C::SR ? d.template d16<i>() : 0u), r.e[1 + CX[i]] + ES * i * POP_GROUP,
+                     ES * r.sy[1 + CY[i]]);
     });
 }
 
@@ -1181,7 +1198,7 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     Nbr nb;
     nb.make(L, x, y, z, n);
     RowAddr ra;
-    if constexpr (ROWS) ra = row_addr<C>(L, x, y, z, n);
+    if constexpr (ROWS) ra = row_addr<C>(L, x, y, z);
     T f[19];
     if (may) {
         if constexpr (ROWS) gather_pre_rows<C>(ra, f);
@@ -1286,11 +1303,12 @@ k_bulk(DevLattice L, StepArgs a, ZMap zm) {
 // least 5 waves per SIMD (96 VGPRs), as the bulk kernel's parity 1 has (the
 // compiler fits them without spilling); the other variants would spill and
 // keep their natural count
-template <class C, int P, bool FORCE, bool SPONGE> constexpr int step_waves() {
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false> constexpr int step_waves() {
+    if (ROWS && P == 1 && !FORCE) return 6;
     return ((std::is_same_v<C, CF16> || std::is_same_v<C, CF32>) && P == 1 && !FORCE && !SPONGE) ? 5 : 1;
 }
 template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
-__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE>())))
+__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE, ROWS>())))
 k_step(DevLattice L, StepArgs a) {
     __shared__ double sh[16][6];
     const uint32_t nsp = L.wall_blocks + L.hw_blocks;
@@ -2235,9 +2253,10 @@ hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent
     return hipGetLastError();
 }
 
-// the row-aligned parity-1 variant where the lattice allows it (16-bit storage)
+// the row-aligned parity-1 variant where the lattice allows it (16- and
+// 32-bit storage)
 template <class C, int P> constexpr bool rows_variant() {
-    return P == 1 && (std::is_same_v<C, CF16> || std::is_same_v<C, CBF16>);
+    return P == 1 && (std::is_same_v<C, CF16> || std::is_same_v<C, CBF16> || std::is_same_v<C, CF32>);
 }
 
 template <class C, int P, bool FORCE, bool SPONGE>

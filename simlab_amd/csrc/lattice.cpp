@@ -552,9 +552,12 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         const char *sw = std::getenv("SIMLAB_CFD_SWIZZLE");
         lat->L.xcd_swizzle = (sw && sw[0] == '0') ? 0u : 1u;
         const char *ra = std::getenv("SIMLAB_CFD_ROWS");
-        const bool half = lat->storage == SIMLAB_STORAGE_FP16 || lat->storage == SIMLAB_STORAGE_BF16;
-        lat->L.rows_aligned =
-            (!(ra && ra[0] == '0') && half && lat->setup.dims[0] % POP_GROUP == 0 && pop_elems(n) < (1ull << 31)) ? 1u : 0u;
+        // 16- and 32-bit storage; every buffer offset of that addressing lies
+        // within one z-plane of populations
+        const uint64_t es = lat->storage == SIMLAB_STORAGE_FP32 ? 4 : lat->storage == SIMLAB_STORAGE_FP64 ? 8 : 2;
+        const uint64_t plane = (uint64_t)lat->setup.dims[0] * lat->setup.dims[1] * 19 * es;
+        lat->L.rows_aligned = (!(ra && ra[0] == '0') && es <= 4 && lat->setup.dims[0] % POP_GROUP == 0 &&
+                               plane < (1ull << 30)) ? 1u : 0u;
     }
     up(lat->rmask, h_mask.data(), (size_t)nrec * 4);
     up(lat->rcell, h_cell.data(), (size_t)nrec * 4);

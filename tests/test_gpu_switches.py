@@ -64,14 +64,15 @@ def test_plain_groups_and_xcd_order_change_nothing(monkeypatch, launch):
     assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
 
 
-@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16])
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.BF16, sa.Storage.FP32])
 @pytest.mark.parametrize("launch,sponge", [("fused", False), ("split", False), ("split", True)])
 def test_row_aligned_addressing_changes_nothing(monkeypatch, storage, launch, sponge):
     # rows of a multiple of 64 cells: the parity-1 step addresses its slots
-    # from three per-lane offsets and scalar shifts (buffer loads and
-    # stores); the same slots, values and dither as the generic addressing,
-    # so bit-identical populations and forces (the rows wrap at y, z = 0 and
-    # the far edge, the periodic sides of the channel)
+    # from three per-lane offsets, one buffer base per z-plane and scalar
+    # row shifts (buffer loads and stores); the same slots, values and
+    # dither as the generic addressing, so bit-identical populations and
+    # forces (the rows wrap at y, z = 0 and the far edge, the periodic sides
+    # of the channel)
     print(f"case rows on/off {storage.name} {launch} sponge={sponge}", flush=True)
     monkeypatch.setenv("SIMLAB_CFD_LAUNCH", launch)
     out = {}
