@@ -678,6 +678,7 @@ template <class T> __device__ __forceinline__ T rwgt(int i) {
 
 // the face cell's macroscopic state from its source's (shader.rs:670-681)
 template <class T> __device__ __forceinline__ void face_rule(const DevLattice &L, uint32_t ty, FaceSrc<T> &s) {
+#pragma clang fp contract(off)
     if (ty == 2u) {
         if (L.total_p) {
             const T ui2 = (T)(L.u_inf[0] * L.u_inf[0] + L.u_inf[1] * L.u_inf[1] + L.u_inf[2] * L.u_inf[2]);
@@ -692,8 +693,12 @@ template <class T> __device__ __forceinline__ void face_rule(const DevLattice &L
     }
 }
 
-// feq_i(rho, u) - w_i + fneq_i(Pi) for one direction (shader.rs:367-396, 682-685)
+// feq_i(rho, u) - w_i + fneq_i(Pi) for one direction (shader.rs:367-396, 682-685).
+// No fused multiply-adds: a face cell's value is then the same whichever
+// lane or loop writes it (a lane per direction, or one lane looping over
+// directions with the weights and velocities folded to constants).
 template <class T> __device__ __forceinline__ T face_value(int i, const FaceSrc<T> &s) {
+#pragma clang fp contract(off)
     const T cx = (T)rcx(i), cy = (T)rcy(i), cz = (T)rcz(i), w = rwgt<T>(i);
     const T cu = cx * s.ux + cy * s.uy + cz * s.uz;
     const T usq = (T)1.5 * (s.ux * s.ux + s.uy * s.uy + s.uz * s.uz);
