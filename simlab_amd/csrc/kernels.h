@@ -156,6 +156,7 @@ struct BatchLaunch {
     int32_t storage;
     bool force, sponge;
     bool fused;
+    bool rows;   // every member has row-aligned addressing (DevLattice::rows_aligned)
 };
 // special kernel of the batch on s_special, bulk kernel on s_bulk (or the
 // one-kernel step on s_special)

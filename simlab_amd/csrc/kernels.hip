@@ -1160,6 +1160,12 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t nb) {
     return (xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q) + j;
 }
 
+// the row-aligned parity-1 variant where the lattice allows it (16- and
+// 32-bit storage)
+template <class C, int P> constexpr bool rows_variant() {
+    return P == 1 && (std::is_same_v<C, CF16> || std::is_same_v<C, CBF16> || std::is_same_v<C, CF32>);
+}
+
 // Every fluid cell with no wall record (face sources and halfway wall cells
 // included). Grid:
 // (xblocks * yblocks, nz); block (bx, by) = `by` whole x-rows (split in
@@ -1345,8 +1351,10 @@ __device__ __forceinline__ uint32_t batch_member(const uint32_t *boff, uint32_t 
 // SPECIAL_BLOCK threads per block; a lattice's tile (bx * by threads, set
 // by the host to divide SPECIAL_BLOCK) fills the block SPECIAL_BLOCK / tile
 // times, so no lattice's blocks idle on another lattice's larger tile
-template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(SPECIAL_BLOCK) k_bulk_batch(const DevLattice *__restrict__ lats,
+// ROWS: every member on row-aligned addressing (parity 1, set by the host)
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(ROWS && !FORCE ? 6 : 1)))
+k_bulk_batch(const DevLattice *__restrict__ lats,
                                                               const uint32_t *__restrict__ boff, uint32_t nl, StepArgs a) {
     const uint32_t i = batch_member(boff, nl, blockIdx.x);
     const DevLattice &L = lats[i];
@@ -1355,7 +1363,7 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_bulk_batch(const DevLattice *
     const uint32_t g = local * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;
     if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
-    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
+    bulk_block<C, P, FORCE, SPONGE, ROWS>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 template <class C, int P, bool FORCE, bool SPONGE>
@@ -1373,8 +1381,8 @@ __global__ void __launch_bounds__(SPECIAL_BLOCK) k_special_batch(const DevLattic
 // SPECIAL_BLOCK). No second stream and no events per step; the host orders
 // the bulk ranges largest member first, so the small members' tiles fill the
 // large member's tail.
-template <class C, int P, bool FORCE, bool SPONGE>
-__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE>())))
+template <class C, int P, bool FORCE, bool SPONGE, bool ROWS = false>
+__global__ void __launch_bounds__(SPECIAL_BLOCK) __attribute__((amdgpu_waves_per_eu(step_waves<C, P, FORCE, SPONGE, ROWS>())))
 k_step_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ boff, uint32_t nl, StepArgs a) {
     __shared__ double sh[16][6];
     const uint32_t nsp = boff[nl];
@@ -1395,7 +1403,7 @@ k_step_batch(const DevLattice *__restrict__ lats, const uint32_t *__restrict__ b
     const uint32_t g = local * (SPECIAL_BLOCK / tt) + threadIdx.x / tt;
     const uint32_t tiles = L.xblocks * L.yblocks;
     if (g >= tiles * L.nz) return;   // whole tiles (whole waves) only
-    bulk_block<C, P, FORCE, SPONGE>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
+    bulk_block<C, P, FORCE, SPONGE, ROWS>(L, a, g % tiles, g / tiles, threadIdx.x % tt);
 }
 
 // ---------------------------------------------------------------- moving geometry
@@ -2258,11 +2266,6 @@ hipError_t launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent
     return hipGetLastError();
 }
 
-// the row-aligned parity-1 variant where the lattice allows it (16- and
-// 32-bit storage)
-template <class C, int P> constexpr bool rows_variant() {
-    return P == 1 && (std::is_same_v<C, CF16> || std::is_same_v<C, CBF16> || std::is_same_v<C, CF32>);
-}
 
 template <class C, int P, bool FORCE, bool SPONGE>
 hipError_t step_launch4(const DevLattice &L, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk, hipEvent_t ev0,
@@ -2473,14 +2476,19 @@ hipError_t t_adjoint_step(const DevLattice &L, const DevLattice &A, int p, const
 
 template <int P, bool FORCE, bool SPONGE>
 hipError_t batch_launch4(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {
+    // row-aligned addressing when every member has it
+    constexpr bool RV = rows_variant<KC, P>();
+    const bool rows = RV && b.rows;
     if (b.fused) {
-        k_step_batch<KC, P, FORCE, SPONGE><<<b.nsp + b.nbulk, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
+        if (rows) k_step_batch<KC, P, FORCE, SPONGE, RV><<<b.nsp + b.nbulk, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
+        else k_step_batch<KC, P, FORCE, SPONGE><<<b.nsp + b.nbulk, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
         return hipGetLastError();
     }
     k_special_batch<KC, P, FORCE, SPONGE><<<b.nsp, SPECIAL_BLOCK, 0, s1>>>(b.lats, b.boff_sp, b.nl, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_bulk_batch<KC, P, FORCE, SPONGE><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
+    if (rows) k_bulk_batch<KC, P, FORCE, SPONGE, RV><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
+    else k_bulk_batch<KC, P, FORCE, SPONGE><<<b.nbulk, SPECIAL_BLOCK, 0, s2>>>(b.lats, b.boff_bulk, b.nl, a);
     return hipGetLastError();
 }
 template <int P> hipError_t batch_launch2(const BatchLaunch &b, const StepArgs &a, hipStream_t s1, hipStream_t s2) {

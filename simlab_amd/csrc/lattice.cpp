@@ -1610,6 +1610,8 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
             BatchLaunch bl;
             bl.lats = b->dlats.as<DevLattice>();
             bl.fused = fused;
+            bl.rows = true;
+            for (size_t i = 0; i < n; ++i) bl.rows = bl.rows && hl[i].rows_aligned != 0;
             if (fused) {
                 bl.boff_sp = b->boffs.as<uint32_t>();
                 bl.boff_bulk = b->boffs.as<uint32_t>() + (n + 1);
