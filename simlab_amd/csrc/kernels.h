@@ -79,7 +79,8 @@ struct DevLattice {
     uint32_t hw_blocks;          // k_special blocks for them
     double *partials;            // [KMAX][wall_blocks + hw_blocks][8] per-block F, T of each step of a chunk
     // x-face sources that feed only their x-face cell leave their post-collision
-    // state here ([2][nz][ny] entries of 12 floats) for k_xfaces after the step
+    // state here ([side][nz][ny] entries of 12 floats, tagged with the
+    // step's mark) for k_xfaces after the step
     float *xsrc;
     uint32_t xdefer;
 
@@ -137,6 +138,9 @@ struct StepArgs {
     float omega[4];
     uint32_t seed;
     uint32_t slot;
+    // this step's tag, unique in the process (never 0): an x-face source's
+    // deferred state carries it, so that only this step's states are taken
+    uint32_t mark_lo, mark_hi;
     // a spinning part: the rotation the lattice's previous voxelization used
     // (vq_valid = 0 when there is none, or it is not known)
     float vq[3][4];
@@ -167,7 +171,7 @@ hipError_t launch_batch_step(const BatchLaunch &b, int parity, const StepArgs &a
 struct StorageLaunchers {
     hipError_t (*init)(const DevLattice &, hipStream_t);
     hipError_t (*faces)(const DevLattice &, int, uint32_t, hipStream_t);
-    hipError_t (*xfaces)(const DevLattice &, int, uint32_t, hipStream_t);
+    hipError_t (*xfaces)(const DevLattice &, int, const StepArgs &, hipStream_t);
     hipError_t (*stream_collide)(const DevLattice &, int, const StepArgs &, hipStream_t, hipStream_t, hipEvent_t,
                                  hipEvent_t);
     hipError_t (*step_fused)(const DevLattice &, int, const StepArgs &, hipStream_t, hipEvent_t, hipEvent_t);
@@ -188,8 +192,9 @@ extern const StorageLaunchers launchers_fp32, launchers_fp16, launchers_bf16, la
 hipError_t launch_init(const DevLattice &L, hipStream_t s);
 // standalone face update before a step of this parity (seed of that step)
 hipError_t launch_faces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
-// the x-face cells from the sources' states the step left in xsrc
-hipError_t launch_xfaces(const DevLattice &L, int parity, uint32_t seed, hipStream_t s);
+// the x-face cells from the sources' states the step (of this parity and
+// tag) left in xsrc
+hipError_t launch_xfaces(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 // k_special on s_special, k_bulk on s_bulk (independent within a step).
 // With ev0/ev1 the bulk kernel's dispatch records its own start and end in
 // them (hipExtLaunchKernel: no extra packets between the kernels)

@@ -771,6 +771,7 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+
 // Wave-cooperative face update after a step of parity Q: for every lane
 // that is a face source, each face cell it feeds is written by 19 lanes of
 // the wave, one direction each. All lanes of the wave must call this.
@@ -1242,7 +1243,8 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
                 float4 *o = (float4 *)(L.xsrc + ((size_t)((x == 1 ? 0u : 1u) * L.nz + z) * L.ny + y) * 12);
                 o[0] = make_float4(s.rho, s.ux, s.uy, s.uz);
                 o[1] = make_float4(s.p[0], s.p[1], s.p[2], s.p[3]);
-                o[2] = make_float4(s.p[4], s.p[5], 0.0f, 0.0f);
+                // the step's tag, as bits
+                *(uint4 *)(o + 2) = make_uint4(__float_as_uint(s.p[4]), __float_as_uint(s.p[5]), a.mark_lo, a.mark_hi);
             }
         }
         const bool coop = src && !xs;
@@ -1251,34 +1253,37 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
 }
 
 // The x-face cells of the sources that left their state in xsrc: one thread
-// per face cell and direction into the domain (the five with c_x = +1 on the
-// -x face, -1 on the +x face), the writes face_writes_wave would have done
-// (same rule, values, slots and dither).
-template <class C, int Q> __global__ void __launch_bounds__(256) k_xfaces(DevLattice L, uint32_t seed) {
+// per face cell, for its five directions into the domain (c_x = +1 on the -x
+// face: 1, 7, 9, 13, 15; c_x = -1 on the +x face: 2, 8, 10, 14, 16), the
+// writes face_writes_wave would have done (same rule, values, slots and
+// dither).
+template <class C, int Q>
+__global__ void __launch_bounds__(256) k_xfaces(DevLattice L, uint32_t seed, uint32_t mark_lo, uint32_t mark_hi) {
     using T = typename C::T;
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t ne = 2ull * L.nz * L.ny;
-    if (g >= ne * 5) return;
-    const uint32_t e = (uint32_t)(g / 5), k = (uint32_t)(g - (uint64_t)e * 5);
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ne = 2u * L.nz * L.ny;
+    if (e >= ne) return;
     const uint32_t side = e / (L.nz * L.ny), r = e - side * L.nz * L.ny;
     const uint32_t z = r / L.ny, y = r - z * L.ny;
-    const uint32_t sx = side ? L.nx - 2 : 1u;
-    // directions 1, 7, 9, 13, 15 have c_x = +1, their opposites 2, 8, 10, 14, 16 c_x = -1
-    const uint32_t i = (k == 0 ? 1u : 5u + 2u * k + (k > 2 ? 2u : 0u)) + side;
     // the source must be a colliding x-only source of the bulk kernel
     if (!(L.per[1] || (y >= 1 && y <= L.ny - 2 && y != 1 && y != L.ny - 2))) return;
     if (!(L.per[2] || (z >= 1 && z <= L.nz - 2 && z != 1 && z != L.nz - 2))) return;
-    const uint32_t fg = L.flags[((size_t)z * L.ny + y) * L.nx + sx];
-    if ((fg & 3u) != 0 || rec_of(fg) != 0) return;
-    const int bx = side ? (int)L.nx - 1 : 0;
-    if (!face_live(L, bx, (int)y, (int)z, (int)i)) return;
+    // a colliding x-only source of this step left its state with this
+    // step's tag (one memory round trip, no flag load)
     const float4 *o = (const float4 *)(L.xsrc + (size_t)e * 12);
-    const float4 a0 = o[0], a1 = o[1], a2 = o[2];
+    const float4 a0 = o[0], a1 = o[1];
+    const uint4 a2 = *(const uint4 *)(o + 2);
+    if (a2.z != mark_lo || a2.w != mark_hi) return;
     FaceSrc<T> fb;
     fb.rho = a0.x; fb.ux = a0.y; fb.uy = a0.z; fb.uz = a0.w;
-    fb.p[0] = a1.x; fb.p[1] = a1.y; fb.p[2] = a1.z; fb.p[3] = a1.w; fb.p[4] = a2.x; fb.p[5] = a2.y;
+    fb.p[0] = a1.x; fb.p[1] = a1.y; fb.p[2] = a1.z; fb.p[3] = a1.w; fb.p[4] = __uint_as_float(a2.x); fb.p[5] = __uint_as_float(a2.y);
     face_rule(L, L.vel_face[side] ? 2u : 3u, fb);
-    face_store<C, Q>(L, bx, (int)y, (int)z, (int)i, face_value((int)i, fb), seed);
+    const int bx = side ? (int)L.nx - 1 : 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int i = (k == 0 ? 1 : 5 + 2 * k + (k > 2 ? 2 : 0)) + (int)side;
+        if (face_live(L, bx, (int)y, (int)z, i)) face_store<C, Q>(L, bx, (int)y, (int)z, i, face_value(i, fb), seed);
+    }
 }
 
 // the sponge variant of parity 1 in 16-bit storage at 5 waves per SIMD like
@@ -2383,14 +2388,15 @@ hipError_t t_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t t_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
+hipError_t t_xfaces(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    const uint32_t seed = a.seed + 2654435761u;   // the face writes' dither seed
     if (!L.xdefer) return hipSuccess;
     if constexpr (std::is_same_v<KC, CF64>) {
         return hipErrorInvalidValue;
     } else {
-        const uint32_t nb = cdiv(2ull * L.nz * L.ny * 5, 256);
-        if (p == 0) k_xfaces<KC, 0><<<nb, 256, 0, s>>>(L, seed);
-        else k_xfaces<KC, 1><<<nb, 256, 0, s>>>(L, seed);
+        const uint32_t nb = cdiv(2ull * L.nz * L.ny, 256);
+        if (p == 0) k_xfaces<KC, 0><<<nb, 256, 0, s>>>(L, seed, a.mark_lo, a.mark_hi);
+        else k_xfaces<KC, 1><<<nb, 256, 0, s>>>(L, seed, a.mark_lo, a.mark_hi);
         return hipGetLastError();
     }
 }
@@ -2541,8 +2547,8 @@ const StorageLaunchers &kl(int32_t storage) {
 
 hipError_t launch_init(const DevLattice &L, hipStream_t s) { return kl(L.storage).init(L, s); }
 hipError_t launch_faces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) { return kl(L.storage).faces(L, p, seed, s); }
-hipError_t launch_xfaces(const DevLattice &L, int p, uint32_t seed, hipStream_t s) {
-    return kl(L.storage).xfaces(L, p, seed, s);
+hipError_t launch_xfaces(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
+    return kl(L.storage).xfaces(L, p, a, s);
 }
 hipError_t launch_stream_collide(const DevLattice &L, int p, const StepArgs &a, hipStream_t s_special, hipStream_t s_bulk,
                                  hipEvent_t ev0, hipEvent_t ev1) {

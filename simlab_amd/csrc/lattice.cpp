@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -280,6 +281,10 @@ uint32_t default_block_threads() {
     return 256;
 }
 
+// step tags, unique in the process (the first is 1; x-face source states
+// start zeroed, so no stale state ever carries a tag a later step gets)
+std::atomic<uint64_t> g_step_mark{0};
+
 void pack_step(const simlab_step_in &s, uint64_t step, uint32_t slot, StepArgs &a) {
     // lattice.rs:675-686: columns of the rotations as f32; seed per step
     for (int k = 0; k < 3; ++k) {
@@ -294,6 +299,9 @@ void pack_step(const simlab_step_in &s, uint64_t step, uint32_t slot, StepArgs &
     a.omega[3] = 0.0f;
     a.seed = (uint32_t)step * 2654435761u;
     a.slot = slot;
+    const uint64_t mark = ++g_step_mark;
+    a.mark_lo = (uint32_t)mark;
+    a.mark_hi = (uint32_t)(mark >> 32);
     std::memset(a.vq, 0, sizeof a.vq);
     a.vq_valid = 0;
 }
@@ -676,6 +684,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
     if (const char *e = std::getenv("SIMLAB_CFD_XDEFER")) L.xdefer = L.xdefer && e[0] != '0';
     if (L.xdefer) {
         lat->xsrc.alloc((size_t)2 * L.nz * L.ny * 12 * sizeof(float), mem);
+        hip_check(hipMemsetAsync(lat->xsrc.p, 0, lat->xsrc.bytes, st), "memset");
         L.xsrc = lat->xsrc.as<float>();
     } else {
         L.xsrc = nullptr;
@@ -818,7 +827,7 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             } else {
                 hip_check(launch_stream_collide(lat->L, par, a, st, st, e0, e1), "stream-collide kernels");
             }
-            if (lat->L.xdefer) hip_check(launch_xfaces(lat->L, par, a.seed + 2654435761u, st), "x-face kernel");
+            if (lat->L.xdefer) hip_check(launch_xfaces(lat->L, par, a, st), "x-face kernel");
             if (s.sample && (lat->nvert || lat->npix)) hip_check(launch_samplers(lat->L, par, a, st), "sampler kernels");
         }
         if (lat->timing) hip_check(hipEventRecord(lat->ev[2 * KMAX + 1], st), "event");
@@ -1665,7 +1674,7 @@ int simlab_cfd_batch_run_still(simlab_batch *b, size_t nsteps, double *ft_out) {
                         }
                         for (size_t i = 0; i < n; ++i)
                             if (hl[i].xdefer)
-                                hip_check(launch_xfaces(hl[i], par, a.seed + 2654435761u, st), "x-face kernel");
+                                hip_check(launch_xfaces(hl[i], par, a, st), "x-face kernel");
                     } else {
                         // special kernel first on the main stream, bulk kernel on
                         // the side stream beside it (as a lattice's own split step)
