@@ -109,6 +109,26 @@ template <class C> __device__ __forceinline__ typename C::E enc(typename C::T v,
     }
 }
 
+// FP16 in its stored scale (x 2^15, the bulk step's scaled arithmetic; the
+// other storages as dec / enc): decoding skips the multiply by 2^-15,
+// encoding the one by 2^15. Both scalings are exact, so a step computed in
+// the scaled units stores the same bits.
+template <class C> __device__ __forceinline__ typename C::T dec_raw(typename C::E e) {
+    if constexpr (std::is_same_v<C, CF16>) return (float)__builtin_bit_cast(_Float16, e);
+    else return dec<C>(e);
+}
+template <class C> __device__ __forceinline__ typename C::E enc_raw(typename C::T v, uint32_t d16) {
+    if constexpr (std::is_same_v<C, CF16>) {
+        float t = __uint_as_float(__float_as_uint(v) + (d16 >> 3));
+        half2_t h = __builtin_amdgcn_cvt_pkrtz(t, t);
+        return __builtin_bit_cast(uint16_t, h[0]);
+    } else {
+        return enc<C>(v, d16);
+    }
+}
+// the population scale of a step in scaled units
+template <class C, bool RAW> constexpr float pop_scale() { return (RAW && std::is_same_v<C, CF16>) ? FS : 1.0f; }
+
 // per-cell dither words: the reference's sround mixing (shader.rs:511-512)
 // applied once to the cell index, then xorshift32 words
 __device__ __forceinline__ uint32_t cell_hash(uint32_t n, uint32_t seed) {
@@ -183,13 +203,16 @@ __device__ __forceinline__ void coords(const DevLattice &L, uint32_t n, uint32_t
 // meant: load_in(p) = gather_pre<p>, store_out(p) = scatter_post<p>,
 // load_prev_out(module p) = gather_post<1-p>, store_prev_out(module p) =
 // scatter_post<1-p, guarded>, store_in(p) = scatter_pre<p> (shader.rs:254-339).
-template <class C, int Q> __device__ __forceinline__ void gather_pre(const DevLattice &L, const Nbr &nb, typename C::T f[19]) {
+template <class C, int Q, bool RAW = false>
+__device__ __forceinline__ void gather_pre(const DevLattice &L, const Nbr &nb, typename C::T f[19]) {
     sfor<19>([&](auto I) {
         constexpr int i = I;
         if constexpr (Q == 0 || i == 0) {
-            f[i] = dec<C>(pop<C>(L, i, nb.n));
+            const typename C::E e = pop<C>(L, i, nb.n);
+            f[i] = RAW ? dec_raw<C>(e) : dec<C>(e);
         } else {
-            f[i] = dec<C>(pop<C>(L, OPP[i], nb.template at<-CX[i], -CY[i], -CZ[i]>()));
+            const typename C::E e = pop<C>(L, OPP[i], nb.template at<-CX[i], -CY[i], -CZ[i]>());
+            f[i] = RAW ? dec_raw<C>(e) : dec<C>(e);
         }
     });
 }
@@ -217,7 +240,7 @@ template <int i> __device__ __forceinline__ bool keeps_inside(const DevLattice &
     return ok;
 }
 
-template <class C, int Q, bool GUARD, bool RT = false>
+template <class C, int Q, bool GUARD, bool RT = false, bool RAW = false>
 __device__ __forceinline__ void scatter_post(const DevLattice &L, const Nbr &nb, typename C::T f[19], uint32_t seed) {
     // RT: f[i] becomes the value as stored (decoded back), for face gathers
     Dither d;
@@ -226,13 +249,15 @@ __device__ __forceinline__ void scatter_post(const DevLattice &L, const Nbr &nb,
         constexpr int i = I;
         if constexpr (Q == 0 || i == 0) {
             constexpr int j = OPP[i];   // slot direction
-            const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<j>() : 0u);
+            const uint32_t dj = C::SR ? d.template d16<j>() : 0u;
+            const typename C::E e = RAW ? enc_raw<C>(f[i], dj) : enc<C>(f[i], dj);
             pop<C>(L, j, nb.n) = e;
             if constexpr (RT) f[i] = dec<C>(e);
         } else {
             bool ok = true;
             if constexpr (GUARD) ok = keeps_inside<i>(L, nb);
-            const typename C::E e = enc<C>(f[i], C::SR ? d.template d16<i>() : 0u);
+            const uint32_t di = C::SR ? d.template d16<i>() : 0u;
+            const typename C::E e = RAW ? enc_raw<C>(f[i], di) : enc<C>(f[i], di);
             if (ok) pop<C>(L, i, nb.template at<CX[i], CY[i], CZ[i]>()) = e;
             if constexpr (RT) f[i] = dec<C>(e);
         }
@@ -301,24 +326,26 @@ __device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t r, typename C::
     else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int32_t)vo, so, 0);
 }
 // input i of a parity-1 step: slot OPP[i] of cell n - c_i
-template <class C> __device__ __forceinline__ void gather_pre_rows(const RowAddr &r, typename C::T f[19]) {
+template <class C, bool RAW = false> __device__ __forceinline__ void gather_pre_rows(const RowAddr &r, typename C::T f[19]) {
     constexpr uint32_t ES = sizeof(typename C::E);
     sfor<19>([&](auto I) {
         constexpr int i = I;
         // the slot's constant offset goes with the lane offset, where it
         // folds into the instruction's immediate offset
-        f[i] = dec<C>(row_load<C>(r.r[1 - CZ[i]], r.e[1 - CX[i]] + ES * OPP[i] * POP_GROUP, ES * r.sy[1 - CY[i]]));
+        const typename C::E e = row_load<C>(r.r[1 - CZ[i]], r.e[1 - CX[i]] + ES * OPP[i] * POP_GROUP, ES * r.sy[1 - CY[i]]);
+        f[i] = RAW ? dec_raw<C>(e) : dec<C>(e);
     });
 }
 // output i of a parity-1 step: slot i of cell n + c_i (slot 0 of n itself)
-template <class C>
+template <class C, bool RAW = false>
 __device__ __forceinline__ void scatter_post_rows(const RowAddr &r, uint32_t n, const typename C::T f[19], uint32_t seed) {
     constexpr uint32_t ES = sizeof(typename C::E);
     Dither d;
     if constexpr (C::SR) d.init(n, seed);
     sfor<19>([&](auto I) {
         constexpr int i = I;
-        row_store<C>(r.r[1 + CZ[i]], enc<C>(f[i], C::SR ? d.template d16<i>() : 0u), r.e[1 + CX[i]] + ES * i * POP_GROUP,
+        const uint32_t di = C::SR ? d.template d16<i>() : 0u;
+        row_store<C>(r.r[1 + CZ[i]], RAW ? enc_raw<C>(f[i], di) : enc<C>(f[i], di), r.e[1 + CX[i]] + ES * i * POP_GROUP,
                      ES * r.sy[1 + CY[i]]);
     });
 }
@@ -437,10 +464,24 @@ template <class T> __device__ __forceinline__ FaceSrc<T> face_source(const DevLa
 // ---------------------------------------------------------------- collision
 // shader.rs:815-847 (+ Guo forcing :442-455). On exit f holds the
 // post-collision f~ (before streaming), rho and u_out as the record saves.
-template <class T, bool FORCE, bool SPONGE, class Hook>
+// SCALE: f in and out in units of 1 / SCALE (a power of two; the moments
+// are scaled back, the outputs' constants scaled up: the same bits, SCALE
+// times over)
+template <class T, bool FORCE, bool SPONGE, class Hook, int SCALE_LOG2 = 0>
 __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_t y, uint32_t z, T f[19], T &rho_out,
                                         T u_out[3], Hook &&tau_hook, T ppost[6]) {
-    const Mom<T> m = moments(f);
+    static_assert(SCALE_LOG2 == 0 || !SPONGE, "the absorbing layer runs in true units");
+    constexpr T OS = (T)(1u << SCALE_LOG2);
+    Mom<T> m = moments(f);
+    if constexpr (SCALE_LOG2 != 0) {
+        // by ldexp, not a multiply: a multiply could fuse into the stress
+        // terms' multiply-adds below and round them differently
+        static_assert(std::is_same_v<T, float>, "FP32 arithmetic");
+        auto sc = [](float v) { return __builtin_ldexpf(v, -SCALE_LOG2); };
+        m.rho1 = sc(m.rho1); m.jx = sc(m.jx); m.jy = sc(m.jy); m.jz = sc(m.jz);
+        m.sxx = sc(m.sxx); m.syy = sc(m.syy); m.szz = sc(m.szz);
+        m.sxy = sc(m.sxy); m.sxz = sc(m.sxz); m.syz = sc(m.syz);
+    }
     const T rho = (T)1 + m.rho1;
     const T irho = rcp_(rho);
     T ux = m.jx * irho, uy = m.jy * irho, uz = m.jz * irho;
@@ -475,13 +516,14 @@ __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_
     // feq + (1-om) fneq, evaluated per +-pair: the even part is shared
     const T usq = (T)1.5 * (ux * ux + uy * uy + uz * uz);
     const T tr = (p[0] + p[1] + p[2]) * ((T)1 / (T)3);
-    f[0] = wgt<T>(0) * (m.rho1 - rho * usq) + k1 * ((T)4.5 * wgt<T>(0)) * (-tr);
+    f[0] = (OS * wgt<T>(0)) * (m.rho1 - rho * usq) + k1 * ((T)4.5 * OS * wgt<T>(0)) * (-tr);
     sfor<9>([&](auto K) {
         constexpr int i = 2 * K + 1;
         constexpr int o = i + 1;
         const T cu = cdot<i>(ux, uy, uz);
-        const T ev = wgt<T>(i) * (m.rho1 + rho * ((T)4.5 * cu * cu - usq)) + k1 * ((T)4.5 * wgt<T>(i)) * (cpc<i>(p) - tr);
-        const T od = ((T)3 * wgt<T>(i)) * rho * cu;
+        const T ev = (OS * wgt<T>(i)) * (m.rho1 + rho * ((T)4.5 * cu * cu - usq)) +
+                     k1 * ((T)4.5 * OS * wgt<T>(i)) * (cpc<i>(p) - tr);
+        const T od = ((T)3 * OS * wgt<T>(i)) * rho * cu;
         f[i] = ev + od;
         f[o] = ev - od;
     });
@@ -513,7 +555,7 @@ __device__ __forceinline__ void collide(const DevLattice &L, uint32_t x, uint32_
             constexpr int i = I;
             const T cu = cdot<i>(ux, uy, uz);
             const T cg = cdot<i>(Gx, Gy, Gz);
-            f[i] = f[i] + wgt<T>(i) * (fo * ((T)3 * (cg - uG) + (T)9 * cu * cg) - (T)1.5 * ((T)1 - om) * cg);
+            f[i] = f[i] + (OS * wgt<T>(i)) * (fo * ((T)3 * (cg - uG) + (T)9 * cu * cg) - (T)1.5 * ((T)1 - om) * cg);
         });
     }
     rho_out = rho;
@@ -920,10 +962,12 @@ __device__ __forceinline__ void halfway_fp(const DevLattice &L, const Nbr &nb, t
     });
 }
 
-template <class C, int P>
+// OS: f in units of 1 / OS (the scaled step)
+template <class C, int P, int SCALE_LOG2 = 0>
 __device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArgs &a, const Nbr &nb, uint32_t m,
                                               typename C::T f[19]) {
     using T = typename C::T;
+    constexpr T OS = (T)(1u << SCALE_LOG2);
     T fp[19];
     halfway_fp<C, P>(L, nb, fp);
     sfor<18>([&](auto I) {
@@ -932,7 +976,8 @@ __device__ __forceinline__ void halfway_links(const DevLattice &L, const StepArg
             const T wi = wgt<T>(i);
             const HalfwayLink<T> h = halfway_link<i, T>(L, a, nb);
             const T fpi = fp[i] + wi;
-            f[OPP[i]] = (fpi - (T)6 * wi * h.cu) - wi;
+            const T v = (fpi - (T)6 * wi * h.cu) - wi;
+            f[OPP[i]] = SCALE_LOG2 ? v * OS : v;
         }
     });
 }
@@ -1211,10 +1256,14 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     nb.make(L, x, y, z, n);
     RowAddr ra;
     if constexpr (ROWS) ra = row_addr<C>(L, x, y, z);
+    // FP16 without the absorbing layer steps in the stored scale (x 2^15):
+    // no multiply per load or store, the same bits
+    constexpr bool RAW = std::is_same_v<C, CF16> && !SPONGE;
+    constexpr int SL = RAW ? 15 : 0;
     T f[19];
     if (may) {
-        if constexpr (ROWS) gather_pre_rows<C>(ra, f);
-        else gather_pre<C, P>(L, nb, f);
+        if constexpr (ROWS) gather_pre_rows<C, RAW>(ra, f);
+        else gather_pre<C, P, RAW>(L, nb, f);
     }
     // ours: fluid cells without a record (plain or halfway wall cells);
     // solid, face and wall-record cells are not
@@ -1222,10 +1271,10 @@ __device__ __forceinline__ void bulk_block(const DevLattice &L, const StepArgs &
     const uint32_t hwm = (fg & FLAG_HALFWAY) ? ((fg >> 1) & 0x7FFFEu) : 0u;
     T rho, u[3], pp[6];
     if (mine) {
-        if (hwm) halfway_links<C, P>(L, a, nb, hwm, f);
-        collide<T, FORCE, SPONGE>(L, x, y, z, f, rho, u, NoHook{}, pp);
-        if constexpr (ROWS) scatter_post_rows<C>(ra, n, f, a.seed);
-        else scatter_post<C, P, false>(L, nb, f, a.seed);
+        if (hwm) halfway_links<C, P, SL>(L, a, nb, hwm, f);
+        collide<T, FORCE, SPONGE, NoHook, SL>(L, x, y, z, f, rho, u, NoHook{}, pp);
+        if constexpr (ROWS) scatter_post_rows<C, RAW>(ra, n, f, a.seed);
+        else scatter_post<C, P, false, false, RAW>(L, nb, f, a.seed);
     }
     const bool src = mine && is_face_source(L, x, y, z);
     if (__ballot(src)) {
