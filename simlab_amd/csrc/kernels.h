@@ -107,7 +107,10 @@ struct DevLattice {
     const uint32_t *occ;
     uint32_t ntri;
     const float4 *tris;          // ntri x 3 (v0, e1, e2)
-    uint32_t *qscr;              // [nrec * 9 * 3]
+    // link hits of a spinning part's triangles, [2][nrec * 9][2]: per half
+    // (StepArgs::hbuf, the step's parity), box cell and odd direction the
+    // nearest and farthest hit's f32 bits (0x7f7fffff and 0 when none)
+    uint32_t *qscr;
     // body-frame cells of one lattice spacing (from near_lo), two bit planes
     // of near_words words each (x fastest): plane 0 marks the cells within
     // NEAR_CELLS of the part's surface or inside it (a box cell outside them
@@ -145,6 +148,11 @@ struct StepArgs {
     // (vq_valid = 0 when there is none, or it is not known)
     float vq[3][4];
     uint32_t vq_valid;
+    // a spinning part: the half of the hit scratch that holds this step's
+    // hits (the step's parity), and whether k_voxelize clears the other
+    // half's entries of the cells near the part at rotation vq (the hits of
+    // the step before, voxelized earlier in the same call)
+    uint32_t hbuf, hclear;
 };
 
 // A batch of lattices stepped together (device arrays prepared by the host):
@@ -212,11 +220,11 @@ hipError_t launch_bulk_planes(const DevLattice &L, int parity, const StepArgs &a
                               hipEvent_t ev1 = nullptr);
 hipError_t launch_macros(const DevLattice &L, int parity, float4 *out, hipStream_t s);
 hipError_t launch_mass(const DevLattice &L, int parity, double *partials, uint32_t nblocks, double *out, hipStream_t s);
-// re-voxelization of a spinning part for a step (tri_scatter, voxelize, qclear)
-hipError_t launch_spin(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
-// the three parts of launch_spin: triangle hits of a step's rotation into
-// the hit scratch, the box cells' records from them, and the clearing of
-// the entries near the part afterwards
+// re-voxelization of a spinning part for a step: triangle hits of the
+// step's rotation into its half of the hit scratch; the box cells' records
+// from them (and the clearing of the other half's entries the step before
+// left, when a.hclear); the clearing of a step's own entries (the last step
+// of a call, so that the scratch is clear between calls)
 hipError_t launch_scatter(const DevLattice &L, const StepArgs &a, hipStream_t s);
 hipError_t launch_voxelize(const DevLattice &L, int parity, const StepArgs &a, hipStream_t s);
 hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_t s);

@@ -1531,20 +1531,31 @@ __device__ __forceinline__ void box_cell(const DevLattice &L, uint32_t k, int &p
     pc = v3((float)px - c.x, (float)py - c.y, (float)pz - c.z);
 }
 
+// a cleared hit entry: no nearest hit (above every hit's bits, which are
+// those of a t in [0, 1]), farthest 0
+__device__ __forceinline__ void hits_clear(uint32_t *o) {
+#pragma unroll
+    for (int w = 0; w < 9; ++w) *(uint2 *)(o + 2 * w) = make_uint2(0x7f7fffffu, 0u);
+}
+
 #if SIMLAB_KSTORE < 0
-// one wave per triangle; its lanes share the lattice cells of the
-// triangle's bounding box (grown by 1.415). Moller-Trumbore for the link
-// segment p + s c_i, s in [0, 1] (shader.rs ray_tri), split into what
-// depends on the triangle and the direction only (pv = c_i x e2, e1 . pv and
-// its reciprocal, n . c_i: once per wave) and what depends on the cell
-// (tv = p - v0, tv x e1 and its dot with e2: once per cell), each the same
-// float operation on the same operands in the same order as the test
-// written out per link, so the hits and q bits are those of the reference's
-// arithmetic.
+// TRI_LANES lanes per triangle, four triangles per wave; a triangle's lanes
+// share the lattice cells of its bounding box (grown by 1.415, as
+// shader.rs:915-927). After subdivision a triangle spans about a cell, so
+// that box holds 34 cells on average and at most about 80: a whole wave per
+// triangle left half its lanes idle and repeated the triangle's setup on
+// all 64. Moller-Trumbore for the link segment p + s c_i, s in [0, 1]
+// (shader.rs ray_tri), split into what depends on the triangle and the
+// direction only (pv = c_i x e2, e1 . pv and its reciprocal, n . c_i: once
+// per lane) and what depends on the cell (tv = p - v0, tv x e1 and its dot
+// with e2: once per cell), each the same float operation on the same
+// operands in the same order as the test written out per link, so the hits
+// and q bits are those of the reference's arithmetic.
+constexpr uint32_t TRI_LANES = 16;
 __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
 #pragma clang fp contract(off)
-    const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
-    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t t = blockIdx.x * (256 / TRI_LANES) + threadIdx.x / TRI_LANES;
+    const uint32_t lane = threadIdx.x % TRI_LANES;
     if (t >= L.ntri) return;
     const float4 t0 = L.tris[3 * t], t1 = L.tris[3 * t + 1], t2 = L.tris[3 * t + 2];
     const V3f A = centerf(L) + from_body(a.r, v3(t0.x, t0.y, t0.z));
@@ -1584,7 +1595,7 @@ __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
         }
     }
     const uint32_t ex = (uint32_t)(hi[0] - lo[0] + 1), ey = (uint32_t)(hi[1] - lo[1] + 1), ez = (uint32_t)(hi[2] - lo[2] + 1);
-    for (uint32_t c = lane; c < ex * ey * ez; c += 64) {
+    for (uint32_t c = lane; c < ex * ey * ez; c += TRI_LANES) {
         const int x = lo[0] + (int)(c % ex), y = lo[1] + (int)((c / ex) % ey), z = lo[2] + (int)(c / (ex * ey));
         const V3f p = v3((float)x, (float)y, (float)z);
         const V3f tv = p - A;
@@ -1611,24 +1622,21 @@ __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
             // max(t, 0), with +0 for t = -0: the hit bits are ordered as
             // unsigned integers, which sort non-negative floats only
             tt = tt > 0.0f ? tt : 0.0f;
-            uint32_t *o = L.qscr + ((size_t)k * 9 + j) * 3;
+            uint32_t *o = L.qscr + (((size_t)a.hbuf * L.nrec + k) * 9 + j) * 2;
             const uint32_t tb = __float_as_uint(tt);
             atomicMin(o, tb);
             atomicMax(o + 1, tb);
-            atomicAdd(o + 2, 1u);
         }
     }
 }
 
 __global__ void __launch_bounds__(256) k_qclear(DevLattice L) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= L.nrec * 9u) return;
-    L.qscr[3 * (size_t)k] = 0x7f7fffffu;
-    L.qscr[3 * (size_t)k + 1] = 0u;
-    L.qscr[3 * (size_t)k + 2] = 0u;
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 2ull * L.nrec * 9u) return;
+    *(uint2 *)(L.qscr + 2 * k) = make_uint2(0x7f7fffffu, 0u);
 }
-// after a step: only cells near the part can hold hits of this step's
-// triangles (every other entry is still clear from the step before)
+// the entries of a step's half that its triangles can have hit: only cells
+// near the part at its rotation (every other entry is still clear)
 __global__ void __launch_bounds__(256) k_qclear_near(DevLattice L, StepArgs a) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= L.nrec) return;
@@ -1636,13 +1644,7 @@ __global__ void __launch_bounds__(256) k_qclear_near(DevLattice L, StepArgs a) {
     V3f pc;
     box_cell(L, k, px, py, pz, pc);
     if (!near_part(L, to_body(a.r, pc))) return;
-    uint32_t *o = L.qscr + (size_t)k * 27;
-#pragma unroll
-    for (int w = 0; w < 9; ++w) {
-        o[3 * w] = 0x7f7fffffu;
-        o[3 * w + 1] = 0u;
-        o[3 * w + 2] = 0u;
-    }
+    hits_clear(L.qscr + ((size_t)a.hbuf * L.nrec + k) * 18);
 }
 #endif
 
@@ -1686,6 +1688,9 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
     T *rmac = (T *)L.rmac;
     T *rft = (T *)L.rft;
     const V3f bnow = to_body(a.r, pc);
+    // the step before's hits (the other half) are all read: clear its
+    // entries near the part then, for the next step's triangles
+    if (a.hclear && near_part(L, to_body(a.vq, pc))) hits_clear(L.qscr + ((size_t)(a.hbuf ^ 1u) * NR + k) * 18);
     if (!near_part(L, bnow)) {
         // far from the part: fluid now and before, no solid neighbour, no
         // hit; the outcome of the full rule below is known. Farther still
@@ -1731,31 +1736,33 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
         return;
     }
     // link hits of this step: a link is kept at the cell it leaves in the
-    // positive direction; the cell at the other end reads it mirrored
+    // positive direction; the cell at the other end reads it mirrored. The
+    // reference also counts the hits (shader.rs:998-1016) and asks for two
+    // of them below; with one hit the nearest and farthest are the same
+    // value, so that their distance alone decides, and the count is not kept
+    const uint32_t *hits = L.qscr + (size_t)a.hbuf * NR * 18;
     float qv[19], qx[19];
-    uint32_t nh[19];
+    bool hit[19];
     for (int i = 1; i < 19; ++i) {
         qv[i] = 9.0f;
         qx[i] = -1.0f;
-        nh[i] = 0u;
+        hit[i] = false;
         if (i & 1) {
-            const uint32_t *o = L.qscr + ((size_t)k * 9 + (i - 1) / 2) * 3;
-            const uint32_t cnt = o[2];
-            if (cnt > 0u) {
-                qv[i] = __uint_as_float(o[0]);
-                qx[i] = __uint_as_float(o[1]);
-                nh[i] = cnt;
+            const uint2 o = *(const uint2 *)(hits + ((size_t)k * 9 + (i - 1) / 2) * 2);
+            if (o.x != 0x7f7fffffu) {
+                qv[i] = __uint_as_float(o.x);
+                qx[i] = __uint_as_float(o.y);
+                hit[i] = true;
             }
         } else {
             const int qx_ = (int)lx + rcx(i), qy_ = (int)ly + rcy(i), qz_ = (int)lz + rcz(i);
             if (qx_ >= 0 && qy_ >= 0 && qz_ >= 0 && qx_ < (int)BX && qy_ < (int)BY && qz_ < (int)BZ) {
                 const uint32_t k2 = ((uint32_t)qz_ * BY + (uint32_t)qy_) * BX + (uint32_t)qx_;
-                const uint32_t *o = L.qscr + ((size_t)k2 * 9 + (i - 2) / 2) * 3;
-                const uint32_t cnt = o[2];
-                if (cnt > 0u) {
-                    qv[i] = 1.0f - __uint_as_float(o[1]);
-                    qx[i] = 1.0f - __uint_as_float(o[0]);
-                    nh[i] = cnt;
+                const uint2 o = *(const uint2 *)(hits + ((size_t)k2 * 9 + (i - 2) / 2) * 2);
+                if (o.x != 0x7f7fffffu) {
+                    qv[i] = 1.0f - __uint_as_float(o.y);
+                    qx[i] = 1.0f - __uint_as_float(o.x);
+                    hit[i] = true;
                 }
             }
         }
@@ -1765,9 +1772,9 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
         // a wall where the neighbour is solid, or where the link crosses a
         // sheet thinner than a cell (two hits); q from the nearest hit
         if (solid_at(L, a, px + rcx(i), py + rcy(i), pz + rcz(i), false)) {
-            if (nh[i] == 0u) qv[i] = 1.0f;
+            if (!hit[i]) qv[i] = 1.0f;
             mask |= 1u << i;
-        } else if (nh[i] >= 2u && qx[i] - qv[i] > 1e-5f) {
+        } else if (qx[i] - qv[i] > 1e-5f) {
             mask |= 1u << i;
         } else {
             qv[i] = 1.0f;
@@ -2642,7 +2649,7 @@ hipError_t launch_batch_step(const BatchLaunch &b, int p, const StepArgs &a, hip
     return kl(b.storage).batch_step(b, p, a, s1, s2);
 }
 hipError_t launch_scatter(const DevLattice &L, const StepArgs &a, hipStream_t s) {
-    if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 4), 256, 0, s>>>(L, a);
+    if (L.ntri) k_tri_scatter<<<cdiv(L.ntri, 256 / TRI_LANES), 256, 0, s>>>(L, a);
     return hipGetLastError();
 }
 hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_t s) {
@@ -2652,14 +2659,8 @@ hipError_t launch_qclear_near(const DevLattice &L, const StepArgs &a, hipStream_
 hipError_t launch_voxelize(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
     return kl(L.storage).voxelize(L, p, a, s);
 }
-hipError_t launch_spin(const DevLattice &L, int p, const StepArgs &a, hipStream_t s) {
-    hipError_t e = launch_scatter(L, a, s);
-    if (e == hipSuccess) e = launch_voxelize(L, p, a, s);
-    if (e == hipSuccess) e = launch_qclear_near(L, a, s);
-    return e;
-}
 hipError_t launch_qclear(const DevLattice &L, hipStream_t s) {
-    k_qclear<<<cdiv((uint64_t)L.nrec * 9, 256), 256, 0, s>>>(L);
+    k_qclear<<<cdiv(2ull * L.nrec * 9, 256), 256, 0, s>>>(L);
     return hipGetLastError();
 }
 

@@ -304,6 +304,8 @@ void pack_step(const simlab_step_in &s, uint64_t step, uint32_t slot, StepArgs &
     a.mark_hi = (uint32_t)(mark >> 32);
     std::memset(a.vq, 0, sizeof a.vq);
     a.vq_valid = 0;
+    a.hbuf = (uint32_t)(step & 1u);
+    a.hclear = 0;
 }
 
 int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const simlab_probes *pr, simlab_lattice **out) {
@@ -623,7 +625,7 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         uint64_t m2 = 0;
         lat->occ.alloc(g->nocc_words * 4, m2);
         lat->tris.alloc(std::max<uint64_t>(1, g->ntri) * 3 * sizeof(float4), m2);
-        lat->qscr.alloc((size_t)lat->nrec * 27 * 4, m2);
+        lat->qscr.alloc((size_t)lat->nrec * 2 * 18 * 4, m2);
         mem += m2;
         hip_check(hipMemcpyAsync(lat->occ.p, g->occ_bits, g->nocc_words * 4, hipMemcpyHostToDevice, st), "upload");
         if (g->ntri)
@@ -672,6 +674,11 @@ int new_lattice(const simlab_lattice_setup *su, const simlab_geometry *g, const 
         if (!hw.empty()) hip_check(hipMemcpyAsync(lat->hw_cells.p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st), "upload");
         L.hw_cells = lat->hw_cells.as<uint32_t>();
     }
+    // a spinning part's special kernel covers its whole swept box (2.1 M
+    // cells for the 384^3 prop): beside the bulk kernel it ran until that one
+    // ended or later, alone before it 85-100 us; the two-kernel step then
+    // puts both on the main stream, 1 % faster in FP16 and FP32
+    lat->concurrent = !lat->moving;
     if (const char *e = std::getenv("SIMLAB_CFD_SERIAL")) lat->concurrent = !(e[0] == '1');
     lat->partials.alloc((size_t)KMAX * (L.wall_blocks + L.hw_blocks) * 8 * 8, mem);
     L.partials = lat->partials.as<double>();
@@ -767,9 +774,10 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
             hipEvent_t e0 = lat->timing ? lat->ev[2 * k] : nullptr, e1 = lat->timing ? lat->ev[2 * k + 1] : nullptr;
             // a spinning part is re-voxelized for this step's rotation first.
             // Its triangle hits come from stream3 when the previous step
-            // launched them there; after the voxelization, stream3 clears
-            // this step's hits and finds the next step's while this step's
-            // stream-collide kernels run (none of them reads the hit scratch)
+            // launched them there; after the voxelization (which also clears
+            // the previous step's hits), stream3 finds the next step's in the
+            // other half of the hit scratch while this step's stream-collide
+            // kernels run (none of them reads the hit scratch)
             if (lat->moving) {
                 if (lat->scat_ready) {
                     hip_check(hipStreamWaitEvent(st, lat->ev_scat, 0), "event wait");
@@ -783,6 +791,10 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                     std::memcpy(a.vq, lat->vox_prev, sizeof a.vq);
                     a.vq_valid = 1;
                 }
+                // the hits of the step before this one in the call, in the
+                // other half of the scratch, are cleared by this voxelization
+                // (the scratch is clear between calls)
+                a.hclear = done + k > 0 ? 1u : 0u;
                 hip_check(launch_voxelize(lat->L, par, a, st), "re-voxelization kernels");
                 std::memcpy(lat->vox_prev, a.r, sizeof lat->vox_prev);
                 lat->vox_prev_valid = true;
@@ -791,11 +803,10 @@ int run_steps(simlab_lattice *lat, const simlab_step_in *steps, size_t nsteps, d
                     pack_step(still ? idn : steps[done + k + 1], step + 1, (uint32_t)(k + 1), an);
                     hip_check(hipEventRecord(lat->ev_vox, st), "event");
                     hip_check(hipStreamWaitEvent(lat->stream3, lat->ev_vox, 0), "event wait");
-                    hip_check(launch_qclear_near(lat->L, a, lat->stream3), "re-voxelization kernels");
                     hip_check(launch_scatter(lat->L, an, lat->stream3), "re-voxelization kernels");
                     hip_check(hipEventRecord(lat->ev_scat, lat->stream3), "event");
                     lat->scat_ready = true;
-                } else {
+                } else if (done + k + 1 == nsteps) {
                     hip_check(launch_qclear_near(lat->L, a, st), "re-voxelization kernels");
                 }
             }

@@ -10,8 +10,9 @@ from simlab_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _prop_run(monkeypatch, overlap):
+def _prop_run(monkeypatch, overlap, launch):
     monkeypatch.setenv("SIMLAB_CFD_SPIN_OVERLAP", "1" if overlap else "0")
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", launch)
     setup, geom, meta = W.spinning_prop(dims=(80, 80, 48), radius=24.0, storage=sa.Storage.FP32)
     lat = sa.Lattice(setup, geom)
     drv = W.SpinDriver(meta["axis"], meta["omega"])
@@ -19,10 +20,11 @@ def _prop_run(monkeypatch, overlap):
     return h, lat.get_pops()
 
 
-def test_spin_scatter_overlap_changes_nothing(monkeypatch):
-    print("case spin overlap on/off", flush=True)
-    h1, p1 = _prop_run(monkeypatch, True)
-    h0, p0 = _prop_run(monkeypatch, False)
+@pytest.mark.parametrize("launch", ["fused", "split"])
+def test_spin_scatter_overlap_changes_nothing(monkeypatch, launch):
+    print(f"case spin overlap on/off, {launch} launch", flush=True)
+    h1, p1 = _prop_run(monkeypatch, True, launch)
+    h0, p0 = _prop_run(monkeypatch, False, launch)
     # the same kernels on the same data, only the stream each runs on differs
     assert np.array_equal(h1, h0) and np.array_equal(p1, p0)
 
