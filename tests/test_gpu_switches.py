@@ -87,3 +87,20 @@ def test_row_aligned_addressing_changes_nothing(monkeypatch, storage, launch, sp
         lat = sa.Lattice(setup, geom)
         out[rows] = (lat.run_still(25), lat.get_pops())
     assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
+
+
+@pytest.mark.parametrize("storage", [sa.Storage.FP16, sa.Storage.FP32])
+def test_special_kernel_stream_changes_nothing(monkeypatch, storage):
+    # two-stream split step: the special kernel on the side stream (bulk on
+    # the main one) or the other way round; the same kernels on the same
+    # data, so bit-identical populations and forces
+    print(f"case side special on/off {storage.name}", flush=True)
+    monkeypatch.setenv("SIMLAB_CFD_LAUNCH", "split")
+    monkeypatch.setenv("SIMLAB_CFD_SERIAL", "0")
+    out = {}
+    for side in ("1", "0"):
+        monkeypatch.setenv("SIMLAB_CFD_SIDE_SPECIAL", side)
+        setup, geom, meta = W.sphere_channel((128, 64, 48), 6.0, 30.0, 200.0, 0.05, 0.1, storage, subdiv=3)
+        lat = sa.Lattice(setup, geom)
+        out[side] = (np.concatenate([lat.run_still(7), lat.run_still(19)]), lat.get_pops())
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
