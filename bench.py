@@ -180,22 +180,34 @@ def study_arm(sa, W, steps, batch_steps):
     against FP64; then the arm at 128^3, 192^3, 256^3 and 384^3 in FP16,
     autotuned, stepped one after the other and as one batch."""
     import numpy as np
-    out = {}
-    for name, st in (("fp64", sa.Storage.FP64), ("fp32", sa.Storage.FP32), ("fp16", sa.Storage.FP16)):
+    out, drag = {}, {}
+    runs = (("fp64", sa.Storage.FP64, 2000), ("fp32", sa.Storage.FP32, 2000), ("fp16", sa.Storage.FP16, 2000),
+            ("fp16_calls_2500", sa.Storage.FP16, 2500))
+    for name, st, call in runs:
         setup, geom, meta = W.drone_arm(storage=st)
         lat = sa.Lattice(setup, geom)
         t0 = time.perf_counter()
         hs = []
-        for done in range(0, steps, 2000):
-            hs.append(lat.run_still(min(2000, steps - done)))
+        for done in range(0, steps, call):
+            hs.append(lat.run_still(min(call, steps - done)))
             print(f"# arm {name}: {done + len(hs[-1])} / {steps} steps", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
-        h = np.concatenate(hs)
-        out[name] = {"drag": float(h[steps // 2:, 0].mean()), "mlups": round(setup.cells() * steps / wall / 1e6, 1),
-                     "nrec": lat.nrec, "halfway_cells": lat.halfway_cells}
+        drag[name] = np.concatenate(hs)[:, 0]
+        out[name] = {"drag": float(drag[name][steps // 2:].mean()), "mlups": round(setup.cells() * steps / wall / 1e6, 1),
+                     "nrec": lat.nrec, "halfway_cells": lat.halfway_cells, "steps_per_call": call}
         del lat
-    for name in ("fp32", "fp16"):
+    d64 = drag["fp64"][steps // 2:]
+    for name in ("fp32", "fp16", "fp16_calls_2500"):
         out[name]["drag_err_vs_fp64"] = out[name]["drag"] / out["fp64"]["drag"] - 1.0
+        # the step-by-step drag against FP64's: median and 99th percentile of
+        # the relative difference over the window
+        rel = np.abs(drag[name][steps // 2:] / d64 - 1.0)
+        out[name]["drag_step_err_p50"] = float(np.median(rel))
+        out[name]["drag_step_err_p99"] = float(np.percentile(rel, 99))
+    # how unsteady the drag is: FP64's own mean over 80 % of the window
+    # against the whole window's, and its standard deviation
+    out["fp64"]["window_spread"] = float(drag["fp64"][steps // 2:steps // 2 + 4 * steps // 10].mean() / d64.mean() - 1.0)
+    out["fp64"]["std_over_mean"] = float(d64.std() / d64.mean())
     for label, sizes, k in (("batch", (128, 192, 256, 384), batch_steps),
                             ("batch_small", (64, 80, 96, 112, 128, 64, 96, 128), 4 * batch_steps)):
         lats, cells, tiles = [], 0, {}
