@@ -234,7 +234,7 @@ template <class T> __device__ __forceinline__ void gauss(T rho, const T u[3], co
 
 // mshader.rs:466-477
 template <class T> __device__ __forceinline__ void stress(const Mom<T> &mo, T rho, const T u[3], T p[6]) {
-    const T r3 = mo.rho1 / (T)3.0;
+    const T r3 = mo.rho1 * K<T>::third();
     p[0] = mo.s[0] - rho * u[0] * u[0] - r3;
     p[1] = mo.s[1] - rho * u[1] * u[1] - r3;
     p[2] = mo.s[2] - rho * u[2] * u[2] - r3;
@@ -373,19 +373,23 @@ template <class S> __global__ __launch_bounds__(256) void k_step(DevLevel L, Ste
     }
     const Mom<T> mo = moments<T>(f);
     const T rho = (T)1.0 + mo.rho1;
+    // one reciprocal of rho for its eleven divisions (an IEEE division is
+    // about ten instructions, and this step is bound by its arithmetic in
+    // 16-bit storage): the same values to about an ulp
+    const T irho = (T)1.0 / rho;
     const T G[3] = {rho * (T)L.grav[0], rho * (T)L.grav[1], rho * (T)L.grav[2]};
     T u[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) u[k] = (mo.j[k] + (T)0.5 * G[k]) / rho;
+    for (int k = 0; k < 3; ++k) u[k] = (mo.j[k] + (T)0.5 * G[k]) * irho;
     T p[6];
     stress<T>(mo, rho, u, p);
-    const T tr = (p[0] + p[1] + p[2]) / (T)3.0;
+    const T tr = (p[0] + p[1] + p[2]) * K<T>::third();
     const T dv[6] = {p[0] - tr, p[1] - tr, p[2] - tr, p[3], p[4], p[5]};
     T tau = (T)L.tau0;
     if (L.smag2 > 0.0) {
         const T pp = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2] + (T)2.0 * (dv[3] * dv[3] + dv[4] * dv[4] + dv[5] * dv[5]);
         const T t0 = (T)L.tau0;
-        tau = (T)0.5 * (t0 + sqrt_(t0 * t0 + (T)18.0 * (T)L.smag2 * sqrt_((T)2.0 * pp) / rho));
+        tau = (T)0.5 * (t0 + sqrt_(t0 * t0 + (T)18.0 * (T)L.smag2 * sqrt_((T)2.0 * pp) * irho));
     }
     T sp = 0;
     if (L.n_sponge) {
@@ -394,15 +398,16 @@ template <class S> __global__ __launch_bounds__(256) void k_step(DevLevel L, Ste
         if (sp > (T)0 && st > tau) tau += sp * sp * (st - tau);
     }
     const T omg = (T)1.0 / tau;
+    const T cr = ((T)1.0 - omg) * irho;
     T sg[6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) sg[k] = K<T>::third() + ((T)1.0 - omg) * dv[k] / rho;
+    for (int k = 0; k < 3; ++k) sg[k] = K<T>::third() + cr * dv[k];
 #pragma unroll
-    for (int k = 3; k < 6; ++k) sg[k] = ((T)1.0 - omg) * dv[k] / rho;
+    for (int k = 3; k < 6; ++k) sg[k] = cr * dv[k];
     T m1[3] = {0, 0, 0};
     if (L.forcing) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) m1[k] = (T)0.5 * G[k] / rho;
+        for (int k = 0; k < 3; ++k) m1[k] = (T)0.5 * G[k] * irho;
     }
     gauss<T>(rho, u, sg, m1, f);
     if (L.n_sponge && L.sigma > 0.0 && sp > (T)0) {
