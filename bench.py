@@ -343,8 +343,8 @@ def cpu_model():
 def cpu_baseline(sa, W, target_s=5.0, repeats=3):
     """The CPU oracle on the bench workload: median MLUPS of `repeats` timed
     samples of about target_s seconds each, on the host threads this process
-    may use (OMP_NUM_THREADS where it is set: the CPU share of one GPU on the
-    GPU box; else every core)."""
+    may use (OMP_NUM_THREADS where it is set, as on a machine that gives
+    each GPU its share of the host's cores; else every core)."""
     import statistics
     from oracle import pyoracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)

@@ -238,7 +238,7 @@ int simlab_cfd_lattice_slice_means(simlab_lattice *lat, float *out);
 
 /* Total fluid mass sum(rho) over the cells that collide (fluid, not face,
  * not solid), accumulated in FP64 on the device. New: the reference has no
- * mass check; it is the north star's global conservation check. */
+ * mass check; it is the global conservation check of the FP16/BF16 storages. */
 int simlab_cfd_lattice_mass(simlab_lattice *lat, double *mass);
 
 /* Raw population slots in the reference's order (slot of direction i and
