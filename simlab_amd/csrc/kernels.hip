@@ -1539,19 +1539,20 @@ __device__ __forceinline__ void hits_clear(uint32_t *o) {
 }
 
 #if SIMLAB_KSTORE < 0
-// TRI_LANES lanes per triangle, four triangles per wave; a triangle's lanes
-// share the lattice cells of its bounding box (grown by 1.415, as
+// TRI_LANES lanes per triangle, eight triangles per wave; a triangle's
+// lanes share the lattice cells of its bounding box (grown by 1.415, as
 // shader.rs:915-927). After subdivision a triangle spans about a cell, so
 // that box holds 34 cells on average and at most about 80: a whole wave per
 // triangle left half its lanes idle and repeated the triangle's setup on
-// all 64. Moller-Trumbore for the link segment p + s c_i, s in [0, 1]
-// (shader.rs ray_tri), split into what depends on the triangle and the
-// direction only (pv = c_i x e2, e1 . pv and its reciprocal, n . c_i: once
+// all 64 (on the 384^3 prop, 16 lanes per triangle stepped 0.7 % slower
+// than 8, 32 lanes 1.7 %). Moller-Trumbore for the link segment
+// p + s c_i, s in [0, 1] (shader.rs ray_tri), split into what depends on
+// the triangle and the direction only (pv = c_i x e2, e1 . pv and its reciprocal, n . c_i: once
 // per lane) and what depends on the cell (tv = p - v0, tv x e1 and its dot
 // with e2: once per cell), each the same float operation on the same
 // operands in the same order as the test written out per link, so the hits
 // and q bits are those of the reference's arithmetic.
-constexpr uint32_t TRI_LANES = 16;
+constexpr uint32_t TRI_LANES = 8;
 __global__ void __launch_bounds__(256) k_tri_scatter(DevLattice L, StepArgs a) {
 #pragma clang fp contract(off)
     const uint32_t t = blockIdx.x * (256 / TRI_LANES) + threadIdx.x / TRI_LANES;
