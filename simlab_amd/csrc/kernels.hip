@@ -1090,7 +1090,16 @@ __device__ __forceinline__ void special_block(const DevLattice &L, const StepArg
     T *rft = (T *)L.rft;
     const size_t NR = L.nrec;
     if (r < L.nrec) {
-        const uint32_t n = L.rcell[r];
+        // a spinning part's records are its swept box, x fastest: the cell
+        // follows from r, and the flag load does not wait for a cell load
+        uint32_t n;
+        if (L.moving) {
+            const uint32_t BX = L.box_n[0], BY = L.box_n[1];
+            n = ((r / (BX * BY) + (uint32_t)L.box_lo[2]) * L.ny + (r / BX) % BY + (uint32_t)L.box_lo[1]) * L.nx + r % BX +
+                (uint32_t)L.box_lo[0];
+        } else {
+            n = L.rcell[r];
+        }
         const uint32_t fg = L.flags[n];
         if ((fg & 3u) == 0 && rec_of(fg) == r + 1) {
             uint32_t x, y, z;
