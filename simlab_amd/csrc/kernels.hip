@@ -1698,24 +1698,26 @@ template <class C, int P> __global__ void __launch_bounds__(256) k_voxelize(DevL
     T *rmac = (T *)L.rmac;
     T *rft = (T *)L.rft;
     const V3f bnow = to_body(a.r, pc);
+    // the cell's body-frame position at the previous voxelization, and
+    // whether it was near the part then
+    const V3f bprev = to_body(a.vq, pc);
+    const bool near_prev = a.vq_valid && near_part(L, bprev);
     // the step before's hits (the other half) are all read: clear its
-    // entries near the part then, for the next step's triangles
-    if (a.hclear && near_part(L, to_body(a.vq, pc))) hits_clear(L.qscr + ((size_t)(a.hbuf ^ 1u) * NR + k) * 18);
+    // entries near the part then, for the next step's triangles (hclear
+    // implies vq_valid)
+    if (a.hclear && near_prev) hits_clear(L.qscr + ((size_t)(a.hbuf ^ 1u) * NR + k) * 18);
     if (!near_part(L, bnow)) {
         // far from the part: fluid now and before, no solid neighbour, no
         // hit; the outcome of the full rule below is known. Farther still
         // (outside plane 1) the cell steps as a plain fluid cell in the
         // bulk kernel and its record is left as it is.
         const bool in1 = near_part(L, bnow, 1);
-        if (a.vq_valid) {
+        if (a.vq_valid && !near_prev) {
             // far at the previous voxelization too: its links, q and F, T
             // were reset then (or at an earlier one) and nothing has written
             // them since; only the flag may change (plane 1)
-            const V3f bprev = to_body(a.vq, pc);
-            if (!near_part(L, bprev)) {
-                if (in1 != near_part(L, bprev, 1)) L.flags[n] = in1 ? (k + 1u) << 2 : 0u;
-                return;
-            }
+            if (in1 != near_part(L, bprev, 1)) L.flags[n] = in1 ? (k + 1u) << 2 : 0u;
+            return;
         }
 #pragma unroll
         for (int w = 0; w < 9; ++w) L.rq[(size_t)w * NR + k] = 0x3C003C00u;
